@@ -1,0 +1,139 @@
+/*
+ * qbh -- MI355X (gfx950) Hamiltonian-application engine for plane-wave DFT/TDDFT.
+ *
+ * C ABI.  Plain pointers and sizes only; every call returns an int status (0 = ok,
+ * negative = error kind, message via qbh_last_error) and no exception crosses it.
+ *
+ * Memory: arguments named *_dev are device pointers (hipMalloc'd by the caller, or any
+ * device allocation on the context's device); everything else is host memory that is
+ * copied during the call.  Coefficient arrays are complex<double> stored as interleaved
+ * (re, im) doubles, band-major with a leading dimension `ld` in complex elements: band n,
+ * plane wave ig lives at element n*ld + ig -- the layout of qball's ComplexMatrix
+ * SlaterDet::c() (src/math/matrix.h:374-380), whose local leading dimension is mloc.
+ * Real-space grids are x-fastest, index x + np0*(y + np1*z) (FourierTransform.h:165-166).
+ *
+ * All work is asynchronous on the context's stream; qbh_sync() waits for it.
+ */
+#ifndef QBH_H
+#define QBH_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define QBH_OK 0
+#define QBH_ERR_ARG (-1)
+#define QBH_ERR_ALLOC (-2)
+#define QBH_ERR_DEVICE (-3)
+#define QBH_ERR_UNSUPPORTED (-4)
+#define QBH_ERR_BLAS (-5)
+
+/* qbh_hpsi flags */
+#define QBH_NONLOCAL 0x1    /* NonLocalPotential::energy(compute_hpsi=true) term */
+#define QBH_KINETIC 0x2     /* 0.5*|k+G-A|^2 c */
+#define QBH_LOCAL 0x4       /* SlaterDet::rs_mul_add with v_r */
+#define QBH_ALL 0x7
+#define QBH_ACCUMULATE 0x8  /* add into hpsi instead of clearing it first */
+#define QBH_FP32 0x10       /* FFTs and projector GEMMs in FP32 (FP32 accumulate) */
+
+typedef struct qbh_ctx_s* qbh_ctx;
+typedef struct qbh_basis_s* qbh_basis;
+typedef struct qbh_ham_s* qbh_ham;
+
+/* ---- context ------------------------------------------------------------------------ */
+/* stream: a hipStream_t to run on, or NULL for a stream owned by the context. */
+int qbh_ctx_create(int device, void* stream, qbh_ctx* out);
+int qbh_ctx_destroy(qbh_ctx ctx);
+const char* qbh_last_error(qbh_ctx ctx);
+int qbh_sync(qbh_ctx ctx);
+/* Cap (bytes) on the transform workspace the library keeps per context. */
+int qbh_ctx_set_workspace_cap(qbh_ctx ctx, double bytes);
+/* Per-kernel timing with HIP events on the context's stream (0 = off). */
+int qbh_ctx_set_timing(qbh_ctx ctx, int on);
+/* Copies up to `cap` "name total_ms count" lines (newline separated) into buf and
+ * resets the accumulated totals.  Synchronises the stream. */
+int qbh_ctx_timing_report(qbh_ctx ctx, char* buf, int cap);
+
+/* ---- plane-wave basis ------------------------------------------------------------------
+ * Replaces the index maps that FourierTransform(basis, np0, np1, np2) builds from a qball
+ * Basis (FourierTransform.cc:144-526, Basis.cc:302-700).  The caller passes its rods in
+ * its own order (Basis::rod_h/rod_k/rod_lmin/rod_size of the local process row); plane
+ * wave ig of rod r is (h, k, lmin + ig - first(r)).  real != 0 selects Γ half-sphere
+ * storage (Basis::real()); the first rod must then be rod (0,0) starting at l = 0.
+ * b: reciprocal lattice vectors as rows (UnitCell::b(0..2)); kpoint in reduced units. */
+int qbh_basis_create(qbh_ctx ctx, int real, int nrods, const int* rod_h, const int* rod_k,
+                     const int* rod_lmin, const int* rod_size, int np0, int np1, int np2,
+                     const double* b, const double* kpoint, qbh_basis* out);
+int qbh_basis_destroy(qbh_basis basis);
+/* ngw = number of plane waves (Basis::localsize()); ncol = nonzero z-columns (nvec_). */
+int qbh_basis_info(qbh_basis basis, int* ngw, int* ncol, int* nlx);
+
+/* Restates Basis::resize (Basis.cc:302-700) for one process row: rods of the sphere
+ * |k+G|^2 < 2*ecut for cell rows a[0..2] (bohr), ecut in Hartree.  Writes at most
+ * max_rods rods; *nrods receives the count (call with max_rods = 0 to size).  np receives
+ * the Basis::np() FFT sizes.  Integer output, identical to the reference's rods. */
+int qbh_rods_build(const double* a, double ecut, int real, const double* kpoint, int max_rods,
+                   int* nrods, int* rod_h, int* rod_k, int* rod_lmin, int* rod_size, int* np);
+/* Basis::factorizable (Basis.cc:126-154). */
+int qbh_factorizable(int n);
+
+/* ---- transforms (FourierTransform::backward / forward, FourierTransform.h:144-153) ------
+ * backward: f_dev[s] (complex grid, np0*np1*np2) = sum_G z(G) exp(+iG.r) for nf transforms.
+ *   pair != 0 (Γ only): transform s carries bands (2s, 2s+1) of c_dev, as in
+ *   backward(c1, c2, f) (FourierTransform.cc:555-567); odd last band alone.
+ *   pair == 0: transform s carries band s (backward(c, f)).
+ * forward: inverse with 1/N; pair != 0 unpacks two bands (forward(f, c1, c2)),
+ *   pair == 0 writes zvec(+G) (forward(f, c)). */
+int qbh_fft_backward(qbh_basis basis, const double* c_dev, long ld, int nbands, int pair,
+                     double* f_dev, int flags);
+int qbh_fft_forward(qbh_basis basis, const double* f_dev, int nbands, int pair, double* c_dev,
+                    long ld, int flags);
+/* forward of a real grid (ChargeDensity.cc:523-550): c = forward(scale * rho + 0i). */
+int qbh_fft_forward_real(qbh_basis basis, const double* rho_dev, double scale, double* c_dev,
+                         int flags);
+
+/* ---- Hamiltonian ------------------------------------------------------------------------ */
+int qbh_ham_create(qbh_ctx ctx, qbh_basis basis, qbh_ham* out);
+int qbh_ham_destroy(qbh_ham ham);
+/* v_r on the grid of the basis (EnergyFunctional::v_r, EnergyFunctional.h:104). */
+int qbh_ham_set_vloc(qbh_ham ham, const double* v_dev);
+/* Norm-conserving KB projectors (NonLocalPotential::init / update_twnl,
+ * NonLocalPotential.cc:76-258, :261-1521): nsp species; species s has na[s] atoms,
+ * npr[s] projectors with angular momentum lproj[...] and weight wt[...] (Species::wsg);
+ * twnl: per species npr[s] x ngw table (index ig + ngw*ipr); tau: 3*na[s] positions per
+ * species (bohr).  Arrays are concatenated over species.  omega = cell volume. */
+int qbh_ham_set_projectors(qbh_ham ham, int nsp, const int* na, const int* npr,
+                           const int* lproj, const double* wt, const double* twnl,
+                           const double* tau, double omega);
+/* Vector potential A (VectorPotential::value_): kinetic |k+G-A|^2 in the order of
+ * VectorPotential::get_kpgpa2 (VectorPotential.h:84-93) and projector phases with
+ * (k+G-A).tau (NonLocalPotential.cc:1966-1969).  The caller re-tables twnl at |k+G-A|
+ * (vector_potential_changed, EnergyFunctional.cc:2047-2057) with set_projectors. */
+int qbh_ham_set_vector_potential(qbh_ham ham, const double* A);
+
+/* Hψ (EnergyFunctional::energy(psi, compute_hpsi=true, dwf, ...) Hψ part,
+ * EnergyFunctional.cc:1142-1153, 1500, 1639-1701): hpsi = [NL] + [T] + [V_loc] psi.
+ * occ (host, nst entries) feeds Enl (NonLocalPotential.cc:2106-2148); enl may be NULL. */
+int qbh_hpsi(qbh_ham ham, const double* psi_dev, long ld, int nst, const double* occ,
+             double* hpsi_dev, long ldh, int flags, double* enl);
+
+/* ρ(r) += (weight/omega) * occ[n] * |psi_n(r)|^2 over bands with positive factor
+ * (SlaterDet::compute_density, SlaterDet.cc:839-932).  rho_dev is np0*np1*np2 doubles;
+ * the caller zeroes it (ChargeDensity::update_density, ChargeDensity.cc:287-290). */
+int qbh_density(qbh_basis basis, const double* psi_dev, long ld, int nst, const double* occ,
+                double weight, double omega, double* rho_dev, int flags);
+
+/* Gram orthonormalisation in place (SlaterDet::gram, SlaterDet.cc:1043-1143) in FP64:
+ * Γ: S = 2 C^T C - c0 c0^T, complex: S = C^H C; S = L L^T; C <- C L^-T (L^-H). */
+int qbh_gram(qbh_basis basis, double* psi_dev, long ld, int nst);
+
+/* One exponential of the ETRS propagator (ExponentialWavefunctionStepper::exponential,
+ * ExponentialWavefunctionStepper.cc:51-148): psi <- sum_{N<=order} (-i dt)^N/N! H^N psi
+ * in place, complex basis.  flags as qbh_hpsi (precision bit honoured). */
+int qbh_exponential(qbh_ham ham, double* psi_dev, long ld, int nst, double dt, int order,
+                    int flags);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* QBH_H */

@@ -1,0 +1,226 @@
+"""ctypes binding of the qbh C ABI (include/qbh.h).
+
+Device buffers are passed as raw addresses (e.g. ``tensor.data_ptr()`` of a torch tensor
+on ``cuda:0``); torch is only the allocator and stream provider here.  No function in
+this module computes anything itself: every numerical result comes from libqbh.so.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+QBH_NONLOCAL = 0x1
+QBH_KINETIC = 0x2
+QBH_LOCAL = 0x4
+QBH_ALL = 0x7
+QBH_ACCUMULATE = 0x8
+QBH_FP32 = 0x10
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libqbh.so")
+
+# exported symbol -> (restype, argtypes)
+_c_int, _c_long, _c_double, _vp = ctypes.c_int, ctypes.c_long, ctypes.c_double, ctypes.c_void_p
+_ip = ctypes.POINTER(ctypes.c_int)
+_dp = ctypes.POINTER(ctypes.c_double)
+SIGNATURES = {
+    "qbh_ctx_create": (_c_int, [_c_int, _vp, ctypes.POINTER(_vp)]),
+    "qbh_ctx_destroy": (_c_int, [_vp]),
+    "qbh_last_error": (ctypes.c_char_p, [_vp]),
+    "qbh_sync": (_c_int, [_vp]),
+    "qbh_ctx_set_workspace_cap": (_c_int, [_vp, _c_double]),
+    "qbh_ctx_set_timing": (_c_int, [_vp, _c_int]),
+    "qbh_ctx_timing_report": (_c_int, [_vp, ctypes.c_char_p, _c_int]),
+    "qbh_basis_create": (_c_int, [_vp, _c_int, _c_int, _ip, _ip, _ip, _ip, _c_int, _c_int,
+                                  _c_int, _dp, _dp, ctypes.POINTER(_vp)]),
+    "qbh_basis_destroy": (_c_int, [_vp]),
+    "qbh_basis_info": (_c_int, [_vp, _ip, _ip, _ip]),
+    "qbh_rods_build": (_c_int, [_dp, _c_double, _c_int, _dp, _c_int, _ip, _ip, _ip, _ip, _ip,
+                                _ip]),
+    "qbh_factorizable": (_c_int, [_c_int]),
+    "qbh_fft_backward": (_c_int, [_vp, _vp, _c_long, _c_int, _c_int, _vp, _c_int]),
+    "qbh_fft_forward": (_c_int, [_vp, _vp, _c_int, _c_int, _vp, _c_long, _c_int]),
+    "qbh_fft_forward_real": (_c_int, [_vp, _vp, _c_double, _vp, _c_int]),
+    "qbh_ham_create": (_c_int, [_vp, _vp, ctypes.POINTER(_vp)]),
+    "qbh_ham_destroy": (_c_int, [_vp]),
+    "qbh_ham_set_vloc": (_c_int, [_vp, _vp]),
+    "qbh_ham_set_projectors": (_c_int, [_vp, _c_int, _ip, _ip, _ip, _dp, _dp, _dp, _c_double]),
+    "qbh_ham_set_vector_potential": (_c_int, [_vp, _dp]),
+    "qbh_hpsi": (_c_int, [_vp, _vp, _c_long, _c_int, _dp, _vp, _c_long, _c_int, _dp]),
+    "qbh_density": (_c_int, [_vp, _vp, _c_long, _c_int, _dp, _c_double, _c_double, _vp, _c_int]),
+    "qbh_gram": (_c_int, [_vp, _vp, _c_long, _c_int]),
+    "qbh_exponential": (_c_int, [_vp, _vp, _c_long, _c_int, _c_double, _c_int, _c_int]),
+}
+
+_lib = None
+
+
+class QbhError(RuntimeError):
+    pass
+
+
+def lib():
+    """Load libqbh.so (raises if it has not been built: there is no fallback path)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise QbhError(f"{LIB_PATH} is missing; build it with __graft_entry__.build()")
+        L = ctypes.CDLL(LIB_PATH)
+        for name, (res, args) in SIGNATURES.items():
+            f = getattr(L, name)
+            f.restype = res
+            f.argtypes = args
+        _lib = L
+    return _lib
+
+
+def _ia(a):
+    a = np.ascontiguousarray(a, dtype=np.int32)
+    return a, a.ctypes.data_as(_ip)
+
+
+def _da(a):
+    a = np.ascontiguousarray(a, dtype=np.float64)
+    return a, a.ctypes.data_as(_dp)
+
+
+def factorizable(n):
+    return bool(lib().qbh_factorizable(int(n)))
+
+
+def rods_build(a, ecut, real=True, kpoint=(0.0, 0.0, 0.0)):
+    """Basis::resize restated in the library (qbh_rods_build).  Returns dict with
+    rod_h, rod_k, rod_lmin, rod_size (int32 arrays) and np (3 ints)."""
+    L = lib()
+    a_, ap = _da(np.asarray(a, dtype=np.float64).reshape(9))
+    k_, kp = _da(np.asarray(kpoint, dtype=np.float64))
+    n = ctypes.c_int(0)
+    npv = (ctypes.c_int * 3)()
+    rc = L.qbh_rods_build(ap, float(ecut), int(real), kp, 0, ctypes.byref(n), None, None, None,
+                          None, npv)
+    if rc != 0:
+        raise QbhError(f"qbh_rods_build failed ({rc})")
+    arrs = [np.zeros(n.value, dtype=np.int32) for _ in range(4)]
+    ptrs = [x.ctypes.data_as(_ip) for x in arrs]
+    rc = L.qbh_rods_build(ap, float(ecut), int(real), kp, n.value, ctypes.byref(n), *ptrs, npv)
+    if rc != 0:
+        raise QbhError(f"qbh_rods_build failed ({rc})")
+    return dict(rod_h=arrs[0], rod_k=arrs[1], rod_lmin=arrs[2], rod_size=arrs[3],
+                np=[npv[0], npv[1], npv[2]])
+
+
+class Context:
+    def __init__(self, device=0, stream=None):
+        self.h = _vp()
+        rc = lib().qbh_ctx_create(int(device), _vp(stream) if stream else None,
+                                  ctypes.byref(self.h))
+        if rc != 0:
+            raise QbhError(f"qbh_ctx_create failed ({rc})")
+
+    def check(self, rc, what):
+        if rc != 0:
+            msg = lib().qbh_last_error(self.h)
+            raise QbhError(f"{what} failed ({rc}): {msg.decode() if msg else ''}")
+
+    def sync(self):
+        self.check(lib().qbh_sync(self.h), "qbh_sync")
+
+    def set_workspace_cap(self, nbytes):
+        self.check(lib().qbh_ctx_set_workspace_cap(self.h, float(nbytes)), "workspace cap")
+
+    def set_timing(self, on):
+        self.check(lib().qbh_ctx_set_timing(self.h, int(bool(on))), "set_timing")
+
+    def timing_report(self):
+        buf = ctypes.create_string_buffer(1 << 16)
+        lib().qbh_ctx_timing_report(self.h, buf, len(buf))
+        out = {}
+        for line in buf.value.decode().splitlines():
+            name, ms, cnt = line.split()
+            out[name] = (float(ms), int(cnt))
+        return out
+
+    def close(self):
+        if self.h:
+            lib().qbh_ctx_destroy(self.h)
+            self.h = _vp()
+
+
+class Basis:
+    def __init__(self, ctx, real, rod_h, rod_k, rod_lmin, rod_size, np_grid, b, kpoint=(0, 0, 0)):
+        self.ctx = ctx
+        self.h = _vp()
+        self._keep = [_ia(rod_h), _ia(rod_k), _ia(rod_lmin), _ia(rod_size)]
+        b_, bp = _da(np.asarray(b, dtype=np.float64).reshape(9))
+        k_, kp = _da(np.asarray(kpoint, dtype=np.float64))
+        self.np = tuple(int(x) for x in np_grid)
+        rc = lib().qbh_basis_create(ctx.h, int(real), len(self._keep[0][0]),
+                                    *[k[1] for k in self._keep], *self.np, bp, kp,
+                                    ctypes.byref(self.h))
+        ctx.check(rc, "qbh_basis_create")
+        ngw, ncol, nlx = ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
+        lib().qbh_basis_info(self.h, ctypes.byref(ngw), ctypes.byref(ncol), ctypes.byref(nlx))
+        self.ngw, self.ncol, self.nlx = ngw.value, ncol.value, nlx.value
+        self.real = bool(real)
+        self.N = self.np[0] * self.np[1] * self.np[2]
+
+    def backward(self, c_ptr, ld, nbands, pair, f_ptr, flags=0):
+        self.ctx.check(lib().qbh_fft_backward(self.h, c_ptr, ld, nbands, int(pair), f_ptr, flags),
+                       "qbh_fft_backward")
+
+    def forward(self, f_ptr, nbands, pair, c_ptr, ld, flags=0):
+        self.ctx.check(lib().qbh_fft_forward(self.h, f_ptr, nbands, int(pair), c_ptr, ld, flags),
+                       "qbh_fft_forward")
+
+    def forward_real(self, rho_ptr, scale, c_ptr, flags=0):
+        self.ctx.check(lib().qbh_fft_forward_real(self.h, rho_ptr, float(scale), c_ptr, flags),
+                       "qbh_fft_forward_real")
+
+    def density(self, psi_ptr, ld, nst, occ, weight, omega, rho_ptr, flags=0):
+        o_, op = _da(occ)
+        self.ctx.check(lib().qbh_density(self.h, psi_ptr, ld, nst, op, float(weight), float(omega),
+                                         rho_ptr, flags), "qbh_density")
+
+    def gram(self, psi_ptr, ld, nst):
+        self.ctx.check(lib().qbh_gram(self.h, psi_ptr, ld, nst), "qbh_gram")
+
+    def close(self):
+        if self.h:
+            lib().qbh_basis_destroy(self.h)
+            self.h = _vp()
+
+
+class Hamiltonian:
+    def __init__(self, ctx, basis):
+        self.ctx, self.basis = ctx, basis
+        self.h = _vp()
+        ctx.check(lib().qbh_ham_create(ctx.h, basis.h, ctypes.byref(self.h)), "qbh_ham_create")
+
+    def set_vloc(self, v_ptr):
+        self.ctx.check(lib().qbh_ham_set_vloc(self.h, v_ptr), "qbh_ham_set_vloc")
+
+    def set_projectors(self, na, npr, lproj, wt, twnl, tau, omega):
+        keep = [_ia(na), _ia(npr), _ia(lproj), _da(wt), _da(twnl), _da(tau)]
+        self.ctx.check(lib().qbh_ham_set_projectors(self.h, len(keep[0][0]),
+                                                    *[k[1] for k in keep], float(omega)),
+                       "qbh_ham_set_projectors")
+
+    def set_vector_potential(self, A):
+        a_, ap = _da(np.asarray(A, dtype=np.float64))
+        self.ctx.check(lib().qbh_ham_set_vector_potential(self.h, ap), "set_vector_potential")
+
+    def hpsi(self, psi_ptr, ld, nst, occ, hpsi_ptr, ldh, flags=QBH_ALL, want_enl=False):
+        o_, op = _da(occ)
+        enl = ctypes.c_double(0.0)
+        self.ctx.check(lib().qbh_hpsi(self.h, psi_ptr, ld, nst, op, hpsi_ptr, ldh, flags,
+                                      ctypes.byref(enl) if want_enl else None), "qbh_hpsi")
+        return enl.value if want_enl else None
+
+    def exponential(self, psi_ptr, ld, nst, dt, order=4, flags=QBH_ALL):
+        self.ctx.check(lib().qbh_exponential(self.h, psi_ptr, ld, nst, float(dt), int(order),
+                                             flags), "qbh_exponential")
+
+    def close(self):
+        if self.h:
+            lib().qbh_ham_destroy(self.h)
+            self.h = _vp()

@@ -1,0 +1,525 @@
+// C-ABI entry points: transforms, Hamiltonian state, Hψ, density, Gram, exponential.
+#include <hip/hip_runtime.h>
+#include <rocblas/rocblas.h>
+#include <rocsolver/rocsolver.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/qbh.h"
+#include "qbh_ham.h"
+
+using namespace qbh;
+
+namespace qbh {
+Basis* basis_of(qbh_basis b);
+Ctx* ctx_of_basis(qbh_basis b);
+Ctx* ctx_of(qbh_ctx c);
+}  // namespace qbh
+
+struct qbh_ham_s {
+  Ham h;
+};
+
+static int fail(Ctx* c, int code, const std::string& msg) {
+  c->last_error = msg;
+  return code;
+}
+
+static int dev_check(Ctx* c, const char* what) {
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return fail(c, QBH_ERR_DEVICE, std::string(what) + ": " + hipGetErrorString(e));
+  return 0;
+}
+
+__global__ void k_fill0(double* __restrict__ p, long n) {
+  const long i = blockIdx.x * 256L + threadIdx.x;
+  if (i < n) p[i] = 0.0;
+}
+
+static void fill0(Ctx* c, double* p, long n) {
+  if (n <= 0) return;
+  k_fill0<<<(unsigned)((n + 255) / 256), 256, 0, c->stream>>>(p, n);
+}
+
+// kinetic-only term when the local potential is off: h += kin * c
+__global__ void k_kinetic(int ngw, int nst, const double* __restrict__ kin,
+                          const double* __restrict__ psi, long ld, double* __restrict__ h,
+                          long ldh) {
+  const long i = blockIdx.x * 256L + threadIdx.x;
+  if (i >= (long)ngw * nst) return;
+  const long n = i / ngw, ig = i - n * ngw;
+  const double k = kin[ig];
+  const double* c = psi + 2 * (n * ld + ig);
+  double* o = h + 2 * (n * ldh + ig);
+  o[0] += k * c[0];
+  o[1] += k * c[1];
+}
+
+// ------------------------------------------------------------------------------------
+// chunked transform driver: calls body(args) for consecutive chunks of slots
+template <typename F>
+static int for_chunks(Ctx* c, Basis* b, int nslots_total, int prec, F&& body) {
+  if (nslots_total <= 0) return 0;
+  const size_t per = slot_bytes(b, prec);
+  size_t S = std::max<size_t>(1, c->ws_cap / per);
+  S = std::min<size_t>(S, (size_t)nslots_total);
+  S = std::min<size_t>(S, 65535);
+  void* ws = ws_get(c, S * per);
+  if (!ws) return fail(c, QBH_ERR_ALLOC, "transform workspace allocation failed");
+  const size_t csz = prec == PREC_FP64 ? 16 : 8;
+  char* t1 = static_cast<char*>(ws);
+  char* t2 = t1 + S * csz * (size_t)b->g.ncol * b->g.np2;
+  for (int s0 = 0; s0 < nslots_total; s0 += (int)S) {
+    PipelineArgs a{};
+    a.nslots = std::min<int>((int)S, nslots_total - s0);
+    a.prec = prec;
+    a.t1 = t1;
+    a.t2 = t2;
+    if (int r = body(a, s0)) return r;
+  }
+  return 0;
+}
+
+// ------------------------------------------------------------------------------------
+extern "C" int qbh_fft_backward(qbh_basis basis, const double* c_dev, long ld, int nbands,
+                                int pair, double* f_dev, int flags) {
+  if (!basis || !c_dev || !f_dev || nbands <= 0) return QBH_ERR_ARG;
+  Basis* b = basis_of(basis);
+  Ctx* c = ctx_of_basis(basis);
+  hipSetDevice(c->device);
+  if (ld < b->g.ngw) return fail(c, QBH_ERR_ARG, "ld < ngw");
+  if (pair && !b->g.real) return fail(c, QBH_ERR_ARG, "pair transforms need a real basis");
+  const int prec = (flags & QBH_FP32) ? PREC_FP32 : PREC_FP64;
+  const int mode = pair ? SLOT_PAIR : (b->g.real ? SLOT_SINGLE_REAL : SLOT_COMPLEX);
+  const int nslots = pair ? (nbands + 1) / 2 : nbands;
+  const long N = (long)b->g.np0 * b->g.np1 * b->g.np2;
+  return for_chunks(c, b, nslots, prec, [&](PipelineArgs& a, int s0) {
+    a.psi = c_dev;
+    a.ld = ld;
+    a.nst = nbands;
+    a.band0 = pair ? 2 * s0 : s0;
+    a.mode = mode;
+    if (int r = pipeline_bwd_yz(c, b, a)) return r;
+    return pipeline_x_store(c, b, a, f_dev + 2 * (long)s0 * N);
+  });
+}
+
+extern "C" int qbh_fft_forward(qbh_basis basis, const double* f_dev, int nbands, int pair,
+                               double* c_dev, long ld, int flags) {
+  if (!basis || !c_dev || !f_dev || nbands <= 0) return QBH_ERR_ARG;
+  Basis* b = basis_of(basis);
+  Ctx* c = ctx_of_basis(basis);
+  hipSetDevice(c->device);
+  if (ld < b->g.ngw) return fail(c, QBH_ERR_ARG, "ld < ngw");
+  if (pair && !b->g.real) return fail(c, QBH_ERR_ARG, "pair transforms need a real basis");
+  const int prec = (flags & QBH_FP32) ? PREC_FP32 : PREC_FP64;
+  const int mode = pair ? SLOT_PAIR : (b->g.real ? SLOT_SINGLE_REAL : SLOT_COMPLEX);
+  const int nslots = pair ? (nbands + 1) / 2 : nbands;
+  const long N = (long)b->g.np0 * b->g.np1 * b->g.np2;
+  return for_chunks(c, b, nslots, prec, [&](PipelineArgs& a, int s0) {
+    a.psi = nullptr;
+    a.ld = ld;
+    a.nst = nbands;
+    a.band0 = pair ? 2 * s0 : s0;
+    a.mode = mode;
+    if (int r = pipeline_x_load(c, b, a, f_dev + 2 * (long)s0 * N, 0, 1.0)) return r;
+    return pipeline_fwd_zy_accumulate(c, b, a, c_dev, ld, nullptr, 1);
+  });
+}
+
+extern "C" int qbh_fft_forward_real(qbh_basis basis, const double* rho_dev, double scale,
+                                    double* c_dev, int flags) {
+  if (!basis || !rho_dev || !c_dev) return QBH_ERR_ARG;
+  Basis* b = basis_of(basis);
+  Ctx* c = ctx_of_basis(basis);
+  hipSetDevice(c->device);
+  const int prec = (flags & QBH_FP32) ? PREC_FP32 : PREC_FP64;
+  const int mode = b->g.real ? SLOT_SINGLE_REAL : SLOT_COMPLEX;
+  return for_chunks(c, b, 1, prec, [&](PipelineArgs& a, int) {
+    a.psi = nullptr;
+    a.ld = b->g.ngw;
+    a.nst = 1;
+    a.band0 = 0;
+    a.mode = mode;
+    if (int r = pipeline_x_load(c, b, a, rho_dev, 1, scale)) return r;
+    return pipeline_fwd_zy_accumulate(c, b, a, c_dev, b->g.ngw, nullptr, 1);
+  });
+}
+
+// ------------------------------------------------------------------------------------
+static int upload_kin(Ham* h) {
+  Basis* b = h->b;
+  const int ngw = b->g.ngw;
+  std::vector<double> kpg2(ngw), kx(3 * (size_t)ngw);
+  hipMemcpy(kx.data(), b->kpgx, kx.size() * sizeof(double), hipMemcpyDeviceToHost);
+  hipMemcpy(kpg2.data(), b->kpg2, kpg2.size() * sizeof(double), hipMemcpyDeviceToHost);
+  const double* A = h->A;
+  const bool zero = A[0] == 0.0 && A[1] == 0.0 && A[2] == 0.0;
+  const double value2 = A[0] * A[0] + A[1] * A[1] + A[2] * A[2];
+  std::vector<double> kin(ngw);
+  for (int ig = 0; ig < ngw; ++ig) {
+    double k2 = kpg2[ig];
+    if (!zero) {  // VectorPotential::get_kpgpa2 (VectorPotential.h:84-93)
+      k2 = kpg2[ig] + value2;
+      k2 -= 2 * A[0] * kx[ig];
+      k2 -= 2 * A[1] * kx[ngw + ig];
+      k2 -= 2 * A[2] * kx[2 * (size_t)ngw + ig];
+    }
+    kin[ig] = 0.5 * k2;
+  }
+  if (!h->kin && hipMalloc(&h->kin, ngw * sizeof(double)) != hipSuccess) {
+    h->kin = nullptr;
+    return fail(h->ctx, QBH_ERR_ALLOC, "kinetic table allocation failed");
+  }
+  hipMemcpy(h->kin, kin.data(), ngw * sizeof(double), hipMemcpyHostToDevice);
+  return 0;
+}
+
+extern "C" int qbh_ham_create(qbh_ctx ctx, qbh_basis basis, qbh_ham* out) {
+  if (!ctx || !basis || !out) return QBH_ERR_ARG;
+  Ctx* c = ctx_of(ctx);
+  hipSetDevice(c->device);
+  auto* hh = new qbh_ham_s();
+  hh->h.ctx = c;
+  hh->h.b = basis_of(basis);
+  if (int r = upload_kin(&hh->h)) {
+    delete hh;
+    return r;
+  }
+  *out = hh;
+  return QBH_OK;
+}
+
+static void free_proj(Ham* h) {
+  for (void* p : {(void*)h->col_l, (void*)h->col_tw, (void*)h->col_atom, (void*)h->col_fac,
+                  (void*)h->twnl, (void*)h->tau, h->anl64, h->anl32})
+    if (p) hipFree(p);
+  h->col_l = h->col_tw = h->col_atom = nullptr;
+  h->col_fac = h->twnl = h->tau = nullptr;
+  h->anl64 = h->anl32 = nullptr;
+  h->anl64_ok = h->anl32_ok = false;
+  h->P = 0;
+}
+
+extern "C" int qbh_ham_destroy(qbh_ham ham) {
+  if (!ham) return QBH_ERR_ARG;
+  Ham* h = &ham->h;
+  hipSetDevice(h->ctx->device);
+  hipStreamSynchronize(h->ctx->stream);
+  free_proj(h);
+  for (void* p : {(void*)h->v, (void*)h->kin, h->f64, h->f32, h->psi32, h->tmp32, (void*)h->red})
+    if (p) hipFree(p);
+  for (void* p : h->expo) hipFree(p);
+  delete ham;
+  return QBH_OK;
+}
+
+extern "C" int qbh_ham_set_vloc(qbh_ham ham, const double* v_dev) {
+  if (!ham || !v_dev) return QBH_ERR_ARG;
+  Ham* h = &ham->h;
+  hipSetDevice(h->ctx->device);
+  const long N = (long)h->b->g.np0 * h->b->g.np1 * h->b->g.np2;
+  if (!h->v && hipMalloc(&h->v, N * sizeof(double)) != hipSuccess) {
+    h->v = nullptr;
+    return fail(h->ctx, QBH_ERR_ALLOC, "v_r allocation failed");
+  }
+  hipMemcpyAsync(h->v, v_dev, N * sizeof(double), hipMemcpyDeviceToDevice, h->ctx->stream);
+  return dev_check(h->ctx, "set_vloc");
+}
+
+extern "C" int qbh_ham_set_projectors(qbh_ham ham, int nsp, const int* na, const int* npr,
+                                      const int* lproj, const double* wt, const double* twnl,
+                                      const double* tau, double omega) {
+  if (!ham || nsp < 0 || (nsp > 0 && (!na || !npr || !lproj || !wt || !twnl || !tau)) ||
+      !(omega > 0.0))
+    return QBH_ERR_ARG;
+  Ham* h = &ham->h;
+  Ctx* c = h->ctx;
+  hipSetDevice(c->device);
+  hipStreamSynchronize(c->stream);
+  free_proj(h);
+  const int ngw = h->b->g.ngw;
+  h->nsp = nsp;
+  h->na.assign(na, na + nsp);
+  h->npr.assign(npr, npr + nsp);
+  h->omega = omega;
+  std::vector<int> cl, ctw, cat;
+  std::vector<double> cf;
+  int tw_row0 = 0, atom0 = 0, ipr0 = 0;
+  long ntw = 0, ntau = 0;
+  for (int s = 0; s < nsp; ++s) {
+    if (na[s] < 0 || npr[s] < 0) return fail(c, QBH_ERR_ARG, "negative na/npr");
+    for (int ipr = 0; ipr < npr[s]; ++ipr) {
+      const int l = lproj[ipr0 + ipr];
+      if (l < 0 || l > 3) return fail(c, QBH_ERR_ARG, "lproj must be in 0..3");
+      // column p = ia + ipr*na (NonLocalPotential.cc:1997 ipra = ia + ipr*na_block)
+      for (int ia = 0; ia < na[s]; ++ia) {
+        cl.push_back(l);
+        ctw.push_back(tw_row0 + ipr);
+        cat.push_back(atom0 + ia);
+        cf.push_back(wt[ipr0 + ipr] * (1.0 / omega));
+      }
+    }
+    tw_row0 += npr[s];
+    atom0 += na[s];
+    ipr0 += npr[s];
+    ntw += (long)npr[s] * ngw;
+    ntau += 3L * na[s];
+  }
+  h->P = (int)cl.size();
+  if (h->P == 0) return QBH_OK;
+  auto up = [&](auto*& dst, const auto* src, size_t n) {
+    using X = std::remove_pointer_t<std::remove_reference_t<decltype(dst)>>;
+    void* d = nullptr;
+    if (hipMalloc(&d, std::max<size_t>(1, n) * sizeof(X)) != hipSuccess) return false;
+    hipMemcpy(d, src, n * sizeof(X), hipMemcpyHostToDevice);
+    dst = static_cast<X*>(d);
+    return true;
+  };
+  if (!up(h->col_l, cl.data(), cl.size()) || !up(h->col_tw, ctw.data(), ctw.size()) ||
+      !up(h->col_atom, cat.data(), cat.size()) || !up(h->col_fac, cf.data(), cf.size()) ||
+      !up(h->twnl, twnl, (size_t)ntw) || !up(h->tau, tau, (size_t)ntau)) {
+    free_proj(h);
+    return fail(c, QBH_ERR_ALLOC, "projector table allocation failed");
+  }
+  return dev_check(c, "set_projectors");
+}
+
+extern "C" int qbh_ham_set_vector_potential(qbh_ham ham, const double* A) {
+  if (!ham || !A) return QBH_ERR_ARG;
+  Ham* h = &ham->h;
+  hipSetDevice(h->ctx->device);
+  hipStreamSynchronize(h->ctx->stream);
+  std::memcpy(h->A, A, 3 * sizeof(double));
+  h->anl64_ok = h->anl32_ok = false;
+  return upload_kin(h);
+}
+
+// ------------------------------------------------------------------------------------
+static int hpsi_impl(Ham* h, const double* psi, long ld, int nst, const double* occ,
+                     double* hpsi, long ldh, int flags, double* enl) {
+  Ctx* c = h->ctx;
+  Basis* b = h->b;
+  const int ngw = b->g.ngw;
+  if (ld < ngw || ldh < ngw) return fail(c, QBH_ERR_ARG, "leading dimension < ngw");
+  if (psi == hpsi) return fail(c, QBH_ERR_ARG, "psi and hpsi must not alias");
+  const bool fp32 = (flags & QBH_FP32) != 0;
+  const int prec = fp32 ? PREC_FP32 : PREC_FP64;
+  if (!(flags & QBH_ACCUMULATE)) fill0(c, hpsi, 2 * ldh * (long)nst);  // dwf.c().clear()
+  if (enl) *enl = 0.0;
+  // non-local first (EnergyFunctional.cc:1500), then kinetic, then local (:1654-1695)
+  if ((flags & QBH_NONLOCAL) && h->P > 0) {
+    if (int r = nl_apply(h, psi, ld, nst, occ, hpsi, ldh, fp32 ? 1 : 0, enl)) return r;
+  }
+  const bool kin = (flags & QBH_KINETIC) != 0;
+  if (flags & QBH_LOCAL) {
+    if (!h->v) return fail(c, QBH_ERR_ARG, "local potential not set");
+    const int mode = b->g.real ? SLOT_PAIR : SLOT_COMPLEX;
+    const int nslots = b->g.real ? (nst + 1) / 2 : nst;
+    int r = for_chunks(c, b, nslots, prec, [&](PipelineArgs& a, int s0) {
+      a.psi = psi;
+      a.ld = ld;
+      a.nst = nst;
+      a.band0 = b->g.real ? 2 * s0 : s0;
+      a.mode = mode;
+      if (int r = pipeline_bwd_yz(c, b, a)) return r;
+      if (int r = pipeline_x_vmul(c, b, a, h->v)) return r;
+      return pipeline_fwd_zy_accumulate(c, b, a, hpsi, ldh, kin ? h->kin : nullptr, 0);
+    });
+    if (r) return r;
+  } else if (kin) {
+    TimedLaunch tl(c, "kinetic");
+    const long n = (long)ngw * nst;
+    k_kinetic<<<(unsigned)((n + 255) / 256), 256, 0, c->stream>>>(ngw, nst, h->kin, psi, ld,
+                                                                  hpsi, ldh);
+  }
+  return dev_check(c, "hpsi");
+}
+
+extern "C" int qbh_hpsi(qbh_ham ham, const double* psi_dev, long ld, int nst, const double* occ,
+                        double* hpsi_dev, long ldh, int flags, double* enl) {
+  if (!ham || !psi_dev || !hpsi_dev || nst <= 0) return QBH_ERR_ARG;
+  Ham* h = &ham->h;
+  hipSetDevice(h->ctx->device);
+  return hpsi_impl(h, psi_dev, ld, nst, occ, hpsi_dev, ldh, flags, enl);
+}
+
+// ------------------------------------------------------------------------------------
+// density: per-slot factors (fac1, fac2, beta1 = Im c1(G=0), beta2 = Im c2(G=0))
+__global__ void k_dens_facs(int nslots, int band0, int nst, int pair, const double* __restrict__ fac,
+                            const double* __restrict__ psi, long ld, double* __restrict__ out) {
+  const int s = blockIdx.x * 256 + threadIdx.x;
+  if (s >= nslots) return;
+  if (pair) {
+    const int n1 = band0 + 2 * s, n2 = n1 + 1;
+    out[4 * s + 0] = fac[n1];
+    out[4 * s + 1] = n2 < nst ? fac[n2] : 0.0;
+    out[4 * s + 2] = psi[2 * (long)n1 * ld + 1];
+    out[4 * s + 3] = n2 < nst ? psi[2 * (long)n2 * ld + 1] : 0.0;
+  } else {
+    out[s] = fac[band0 + s];
+  }
+}
+
+extern "C" int qbh_density(qbh_basis basis, const double* psi_dev, long ld, int nst,
+                           const double* occ, double weight, double omega, double* rho_dev,
+                           int flags) {
+  if (!basis || !psi_dev || !rho_dev || !occ || nst <= 0 || !(omega > 0.0)) return QBH_ERR_ARG;
+  Basis* b = basis_of(basis);
+  Ctx* c = ctx_of_basis(basis);
+  hipSetDevice(c->device);
+  if (ld < b->g.ngw) return fail(c, QBH_ERR_ARG, "ld < ngw");
+  const int prec = (flags & QBH_FP32) ? PREC_FP32 : PREC_FP64;
+  const double prefac = weight / omega;
+  std::vector<double> fac(nst);
+  for (int n = 0; n < nst; ++n) {
+    const double f = prefac * occ[n];
+    fac[n] = f > 0.0 ? f : 0.0;
+  }
+  double* dfac = nullptr;
+  const bool pair = b->g.real;
+  const int nslots = pair ? (nst + 1) / 2 : nst;
+  if (hipMalloc(&dfac, (nst + 4 * (size_t)nslots) * sizeof(double)) != hipSuccess)
+    return fail(c, QBH_ERR_ALLOC, "density factor allocation failed");
+  hipMemcpyAsync(dfac, fac.data(), nst * sizeof(double), hipMemcpyHostToDevice, c->stream);
+  double* slotf = dfac + nst;
+  int r = for_chunks(c, b, nslots, prec, [&](PipelineArgs& a, int s0) {
+    a.psi = psi_dev;
+    a.ld = ld;
+    a.nst = nst;
+    a.band0 = pair ? 2 * s0 : s0;
+    a.mode = pair ? SLOT_PAIR : SLOT_COMPLEX;
+    k_dens_facs<<<(a.nslots + 255) / 256, 256, 0, c->stream>>>(a.nslots, a.band0, nst, pair ? 1 : 0,
+                                                               dfac, psi_dev, ld,
+                                                               slotf + 4 * (size_t)s0);
+    if (int r = pipeline_bwd_yz(c, b, a)) return r;
+    return pipeline_x_density(c, b, a, slotf + 4 * (size_t)s0, rho_dev);
+  });
+  hipStreamSynchronize(c->stream);
+  hipFree(dfac);
+  if (r) return r;
+  return dev_check(c, "density");
+}
+
+// ------------------------------------------------------------------------------------
+extern "C" int qbh_gram(qbh_basis basis, double* psi_dev, long ld, int nst) {
+  if (!basis || !psi_dev || nst <= 0) return QBH_ERR_ARG;
+  Basis* b = basis_of(basis);
+  Ctx* c = ctx_of_basis(basis);
+  hipSetDevice(c->device);
+  const int ngw = b->g.ngw;
+  if (ld < ngw) return fail(c, QBH_ERR_ARG, "ld < ngw");
+  void* hb;
+  if (int r = blas_handle(c, &hb)) return r;
+  rocblas_handle bh = static_cast<rocblas_handle>(hb);
+  const bool real = b->g.real;
+  double* S = nullptr;
+  rocblas_int* info = nullptr;
+  const size_t ssz = (size_t)nst * nst * (real ? 1 : 2);
+  if (hipMalloc(&S, ssz * sizeof(double)) != hipSuccess ||
+      hipMalloc(&info, sizeof(rocblas_int)) != hipSuccess) {
+    if (S) hipFree(S);
+    return fail(c, QBH_ERR_ALLOC, "gram scratch allocation failed");
+  }
+  rocblas_status st;
+  if (real) {
+    const double two = 2.0, zero = 0.0, mone = -1.0, one = 1.0;
+    // S = 2 C^T C - c0 c0^T (SlaterDet.cc:1065-1066), lower triangle
+    st = rocblas_dsyrk(bh, rocblas_fill_lower, rocblas_operation_transpose, nst, 2 * ngw, &two,
+                       psi_dev, 2 * ld, &zero, S, nst);
+    if (st == rocblas_status_success)
+      st = rocblas_dsyr(bh, rocblas_fill_lower, nst, &mone, psi_dev, 2 * ld, S, nst);
+    if (st == rocblas_status_success)
+      st = rocsolver_dpotrf(bh, rocblas_fill_lower, nst, S, nst, info);
+    if (st == rocblas_status_success)
+      st = rocblas_dtrsm(bh, rocblas_side_right, rocblas_fill_lower, rocblas_operation_transpose,
+                         rocblas_diagonal_non_unit, 2 * ngw, nst, &one, S, nst, psi_dev, 2 * ld);
+  } else {
+    const double one = 1.0, zero = 0.0;
+    const rocblas_double_complex zone(1.0, 0.0);
+    auto* C = reinterpret_cast<rocblas_double_complex*>(psi_dev);
+    auto* Sz = reinterpret_cast<rocblas_double_complex*>(S);
+    st = rocblas_zherk(bh, rocblas_fill_lower, rocblas_operation_conjugate_transpose, nst, ngw,
+                       &one, C, ld, &zero, Sz, nst);
+    if (st == rocblas_status_success)
+      st = rocsolver_zpotrf(bh, rocblas_fill_lower, nst, Sz, nst, info);
+    if (st == rocblas_status_success)
+      st = rocblas_ztrsm(bh, rocblas_side_right, rocblas_fill_lower,
+                         rocblas_operation_conjugate_transpose, rocblas_diagonal_non_unit, ngw,
+                         nst, &zone, Sz, nst, C, ld);
+  }
+  rocblas_int hinfo = 0;
+  hipMemcpyAsync(&hinfo, info, sizeof(rocblas_int), hipMemcpyDeviceToHost, c->stream);
+  hipStreamSynchronize(c->stream);
+  hipFree(S);
+  hipFree(info);
+  if (st != rocblas_status_success)
+    return fail(c, QBH_ERR_BLAS, std::string("gram: ") + rocblas_status_to_string(st));
+  if (hinfo != 0) return fail(c, QBH_ERR_ARG, "gram: overlap matrix not positive definite");
+  return dev_check(c, "gram");
+}
+
+// ------------------------------------------------------------------------------------
+__global__ void k_zaxpy(long n, double ar, double ai, const double* __restrict__ x, long ldx,
+                        double* __restrict__ y, long ldy, int ngw) {
+  const long i = blockIdx.x * 256L + threadIdx.x;
+  if (i >= n) return;
+  const long col = i / ngw, ig = i - col * ngw;
+  const double* xp = x + 2 * (col * ldx + ig);
+  double* yp = y + 2 * (col * ldy + ig);
+  const double xr = xp[0], xi = xp[1];
+  yp[0] += ar * xr - ai * xi;
+  yp[1] += ar * xi + ai * xr;
+}
+
+extern "C" int qbh_exponential(qbh_ham ham, double* psi_dev, long ld, int nst, double dt,
+                               int order, int flags) {
+  if (!ham || !psi_dev || nst <= 0 || order < 1) return QBH_ERR_ARG;
+  Ham* h = &ham->h;
+  Ctx* c = h->ctx;
+  hipSetDevice(c->device);
+  const int ngw = h->b->g.ngw;
+  if (ld < ngw) return fail(c, QBH_ERR_ARG, "ld < ngw");
+  const size_t bytes = 2 * sizeof(double) * (size_t)ngw * nst;
+  if (h->expo.size() != 3 || h->expo_bytes < bytes) {
+    for (void* p : h->expo) hipFree(p);
+    h->expo.assign(3, nullptr);
+    for (auto& p : h->expo)
+      if (hipMalloc(&p, bytes) != hipSuccess) {
+        p = nullptr;
+        return fail(c, QBH_ERR_ALLOC, "exponential buffers allocation failed");
+      }
+    h->expo_bytes = bytes;
+  }
+  double* expwf = static_cast<double*>(h->expo[0]);
+  double* bufs[2] = {static_cast<double*>(h->expo[1]), static_cast<double*>(h->expo[2])};
+  // expwf = psi (compact, ld = ngw)
+  hipMemcpy2DAsync(expwf, 2 * sizeof(double) * ngw, psi_dev, 2 * sizeof(double) * ld,
+                   2 * sizeof(double) * ngw, nst, hipMemcpyDeviceToDevice, c->stream);
+  std::vector<double> occ(nst, 0.0);
+  double fr = 1.0, fi = 0.0;  // factor
+  const double* in = psi_dev;
+  long ldin = ld;
+  const int hflags = (flags & (QBH_ALL | QBH_FP32)) == 0 ? QBH_ALL : (flags & (QBH_ALL | QBH_FP32));
+  for (int N = 1; N <= order; ++N) {
+    // factor *= -i*dt/N (ExponentialWavefunctionStepper.cc:97-100)
+    const double m = dt / double(N);
+    const double nr = fi * m, ni = -fr * m;
+    fr = nr;
+    fi = ni;
+    double* out = bufs[(N - 1) & 1];
+    if (int r = hpsi_impl(h, in, ldin, nst, occ.data(), out, ngw, hflags, nullptr)) return r;
+    const long n = (long)ngw * nst;
+    k_zaxpy<<<(unsigned)((n + 255) / 256), 256, 0, c->stream>>>(n, fr, fi, out, ngw, expwf, ngw,
+                                                                ngw);
+    in = out;
+    ldin = ngw;
+  }
+  hipMemcpy2DAsync(psi_dev, 2 * sizeof(double) * ld, expwf, 2 * sizeof(double) * ngw,
+                   2 * sizeof(double) * ngw, nst, hipMemcpyDeviceToDevice, c->stream);
+  return dev_check(c, "exponential");
+}

@@ -1,0 +1,50 @@
+// Internal Hamiltonian state shared by qbh_api.hip and qbh_nl.hip.
+#pragma once
+#include <vector>
+
+#include "qbh_internal.h"
+
+namespace qbh {
+
+struct Ham {
+  Ctx* ctx = nullptr;
+  Basis* b = nullptr;
+  double* v = nullptr;    // v_r on the grid (owned copy)
+  double* kin = nullptr;  // 0.5 |k+G-A|^2 per plane wave
+  double A[3] = {0, 0, 0};
+  // projectors
+  int nsp = 0;
+  int P = 0;                      // total projector columns sum_s npr[s]*na[s]
+  std::vector<int> na, npr;
+  double omega = 0.0;
+  int* col_l = nullptr;           // per column: l
+  int* col_tw = nullptr;          // per column: row of the twnl table
+  int* col_atom = nullptr;        // per column: atom index into tau
+  double* col_fac = nullptr;      // per column: wt/Omega
+  double* twnl = nullptr;         // concatenated tables, row = species projector
+  double* tau = nullptr;          // 3 * total atoms
+  void* anl64 = nullptr;          // Γ: 2ngw x P real; complex: ngw x P complex
+  void* anl32 = nullptr;
+  bool anl64_ok = false, anl32_ok = false;
+  // scratch (grown on demand)
+  void* f64 = nullptr;            // F (P x nst), real (Γ) or complex
+  size_t f64_bytes = 0;
+  void* f32 = nullptr;
+  size_t f32_bytes = 0;
+  void* psi32 = nullptr;          // FP32 copy of psi / FP32 product
+  size_t psi32_bytes = 0;
+  void* tmp32 = nullptr;
+  size_t tmp32_bytes = 0;
+  double* red = nullptr;          // reduction partials, then occupations
+  size_t red_n = 0;
+  std::vector<double> occ_host;   // occupations currently on the device
+  std::vector<void*> expo;        // exponential buffers
+  size_t expo_bytes = 0;
+};
+
+int nl_apply(Ham* h, const double* psi, long ld, int nst, const double* occ, double* hpsi,
+             long ldh, int fp32, double* enl);
+int blas_handle(Ctx* c, void** out);
+void* grow(void*& p, size_t& have, size_t need);
+
+}  // namespace qbh

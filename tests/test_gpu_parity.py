@@ -1,0 +1,258 @@
+"""GPU parity of the HIP path (through the C ABI) against the CPU oracle.
+
+FP64 bar: relative 2-norm error <= 1e-12 for transforms, Hψ and density; energies
+<= 1e-10 Ha absolute.  FP32-bulk bar: relative error <= 2e-6 (float rounding of the
+transforms and projector GEMMs, accumulated into FP64).
+"""
+import numpy as np
+import pytest
+
+from helpers import (from_dev, make_gpu_basis, make_gpu_ham, max_rel, oracle_for, real_to_dev,
+                     rel_err, to_dev)
+
+pytestmark = pytest.mark.gpu
+
+FP64_TOL = 1e-12
+FP32_TOL = 2e-6
+
+
+@pytest.fixture(scope="module")
+def si8():
+    from qball_amd.workload import make_system
+    return make_system(ncell=1, ecut_ry=25.0, nbands=15, seed=11)
+
+
+@pytest.fixture(scope="module")
+def si8c():
+    from qball_amd.workload import make_system
+    return make_system(ncell=1, ecut_ry=25.0, nbands=8, seed=12, complex_basis=True)
+
+
+@pytest.fixture(scope="module")
+def si64():
+    from qball_amd.workload import make_system
+    return make_system(ncell=2, ecut_ry=25.0, seed=0)
+
+
+def _backward_check(ctx, sysm, pair, flags=0, tol=FP64_TOL):
+    import torch
+    from oracle import fft as offt
+    ob, maps, _ = oracle_for(sysm)
+    basis = make_gpu_basis(ctx, sysm)
+    assert basis.ngw == ob.size
+    nb = sysm.nst
+    c = sysm.psi
+    cd = to_dev(c)
+    nslots = (nb + 1) // 2 if pair else nb
+    f = torch.zeros(2 * nslots * basis.N, dtype=torch.float64, device="cuda")
+    basis.backward(cd.data_ptr(), basis.ngw, nb, pair, f.data_ptr(), flags)
+    ctx.sync()
+    fg = f.cpu().numpy().view(np.complex128).reshape(nslots, -1)
+    errs = []
+    for s in range(nslots):
+        if pair:
+            n1, n2 = 2 * s, 2 * s + 1
+            ref = offt.backward(maps, c[:, n1], c[:, n2] if n2 < nb else None).ravel()
+        else:
+            ref = offt.backward(maps, c[:, s]).ravel()
+        errs.append(rel_err(fg[s], ref))
+    basis.close()
+    assert max(errs) <= tol, errs
+
+
+def test_fft_backward_pair_si8(gpu_ctx, si8):
+    _backward_check(gpu_ctx, si8, pair=True)
+
+
+def test_fft_backward_single_si8(gpu_ctx, si8):
+    _backward_check(gpu_ctx, si8, pair=False)
+
+
+def test_fft_backward_complex_si8(gpu_ctx, si8c):
+    _backward_check(gpu_ctx, si8c, pair=False)
+
+
+def test_fft_backward_fp32_si8(gpu_ctx, si8):
+    _backward_check(gpu_ctx, si8, pair=True, flags=0x10, tol=FP32_TOL)
+
+
+@pytest.mark.parametrize("pair", [True, False])
+def test_fft_forward_si8(gpu_ctx, si8, pair):
+    import torch
+    from oracle import fft as offt
+    ob, maps, _ = oracle_for(si8)
+    basis = make_gpu_basis(gpu_ctx, si8)
+    rng = np.random.default_rng(5)
+    nb = 5
+    nslots = (nb + 1) // 2 if pair else nb
+    f = rng.standard_normal((nslots, basis.N)) + 1j * rng.standard_normal((nslots, basis.N))
+    fd = torch.from_numpy(f.view(np.float64).ravel().copy()).cuda()
+    out = torch.zeros(2 * basis.ngw * nb, dtype=torch.float64, device="cuda")
+    basis.forward(fd.data_ptr(), nb, pair, out.data_ptr(), basis.ngw)
+    gpu_ctx.sync()
+    cg = from_dev(out, basis.ngw, nb)
+    for s in range(nslots):
+        fs = f[s].reshape(maps.np2, maps.np1, maps.np0)
+        if pair:
+            c1, c2 = offt.forward(maps, fs, pair=True)
+            assert rel_err(cg[:, 2 * s], c1) <= FP64_TOL
+            if 2 * s + 1 < nb:
+                assert rel_err(cg[:, 2 * s + 1], c2) <= FP64_TOL
+        else:
+            assert rel_err(cg[:, s], offt.forward(maps, fs)) <= FP64_TOL
+    basis.close()
+
+
+def _hpsi_check(ctx, sysm, flags, tol=FP64_TOL, with_proj=True):
+    import torch
+    from oracle import hpsi as oh
+    ob, maps, sp = oracle_for(sysm)
+    basis = make_gpu_basis(ctx, sysm)
+    ham = make_gpu_ham(ctx, basis, sysm, with_proj=with_proj)
+    c = sysm.psi
+    cd = to_dev(c)
+    hd = torch.zeros_like(cd)
+    enl = ham.hpsi(cd.data_ptr(), basis.ngw, sysm.nst, sysm.occ, hd.data_ptr(), basis.ngw,
+                   flags=flags, want_enl=True)
+    ctx.sync()
+    hg = from_dev(hd, basis.ngw, sysm.nst)
+    ref = np.zeros_like(c)
+    enl_ref = 0.0
+    if flags & 0x1:
+        enl_ref = oh.nonlocal_energy(ob, [sp], c, sysm.occ, sysm.omega, cp=ref)
+    if flags & 0x2:
+        oh.kinetic_add(ob.kpg2, c, ref)
+    if flags & 0x4:
+        oh.rs_mul_add(maps, c, sysm.v, ref)
+    ham.close()
+    basis.close()
+    e = rel_err(hg, ref)
+    assert e <= tol, e
+    if flags & 0x1:
+        etol = 1e-10 if tol == FP64_TOL else 1e-5 * max(1.0, abs(enl_ref))
+        assert abs(enl - enl_ref) <= etol, (enl, enl_ref)
+    return e
+
+
+def test_hpsi_local_si8(gpu_ctx, si8):
+    _hpsi_check(gpu_ctx, si8, 0x4)
+
+
+def test_hpsi_kinetic_only_si8(gpu_ctx, si8):
+    _hpsi_check(gpu_ctx, si8, 0x2)
+
+
+def test_hpsi_nonlocal_si8(gpu_ctx, si8):
+    _hpsi_check(gpu_ctx, si8, 0x1)
+
+
+def test_hpsi_full_si8(gpu_ctx, si8):
+    _hpsi_check(gpu_ctx, si8, 0x7)
+
+
+def test_hpsi_full_complex_si8(gpu_ctx, si8c):
+    _hpsi_check(gpu_ctx, si8c, 0x7)
+
+
+def test_hpsi_full_si64(gpu_ctx, si64):
+    _hpsi_check(gpu_ctx, si64, 0x7)
+
+
+def test_hpsi_fp32_si64(gpu_ctx, si64):
+    _hpsi_check(gpu_ctx, si64, 0x7 | 0x10, tol=FP32_TOL)
+
+
+@pytest.mark.parametrize("which", ["si8", "si8c"])
+def test_density(gpu_ctx, which, request):
+    import torch
+    from oracle import hpsi as oh
+    sysm = request.getfixturevalue(which)
+    ob, maps, _ = oracle_for(sysm)
+    basis = make_gpu_basis(gpu_ctx, sysm)
+    cd = to_dev(sysm.psi)
+    occ = sysm.occ.copy()
+    occ[-1] = 0.0  # a band with zero factor is skipped
+    rho = torch.zeros(basis.N, dtype=torch.float64, device="cuda")
+    basis.density(cd.data_ptr(), basis.ngw, sysm.nst, occ, 1.0, sysm.omega, rho.data_ptr())
+    gpu_ctx.sync()
+    ref = oh.compute_density(maps, sysm.psi, occ, 1.0, sysm.omega)
+    basis.close()
+    assert rel_err(rho.cpu().numpy(), ref) <= FP64_TOL
+
+
+def test_density_rhog_si8(gpu_ctx, si8):
+    """update_density's charge_vft: rhog = forward(Omega*rho) on the density basis."""
+    import torch
+    from oracle import hpsi as oh
+    from oracle.basis import Basis as OB, FFTMaps, UnitCell
+    from qball_amd import Basis
+    from qball_amd import capi
+    ob, maps, _ = oracle_for(si8)
+    vob = OB(UnitCell(si8.a), 4.0 * si8.ecut)
+    vmaps = FFTMaps(vob, *si8.np)
+    rho, nel, rhog = oh.update_density(maps, vmaps, si8.psi, si8.occ, 1.0, si8.omega)
+    r = capi.rods_build(si8.a, 4.0 * si8.ecut, True)
+    vb = Basis(gpu_ctx, True, r["rod_h"], r["rod_k"], r["rod_lmin"], r["rod_size"], si8.np, si8.b)
+    rd = real_to_dev(rho)
+    out = torch.zeros(2 * vb.ngw, dtype=torch.float64, device="cuda")
+    vb.forward_real(rd.data_ptr(), si8.omega, out.data_ptr())
+    gpu_ctx.sync()
+    g = out.cpu().numpy().view(np.complex128)
+    vb.close()
+    assert rel_err(g, rhog) <= FP64_TOL
+
+
+@pytest.mark.parametrize("which", ["si8", "si8c"])
+def test_gram(gpu_ctx, which, request):
+    from oracle import hpsi as oh
+    sysm = request.getfixturevalue(which)
+    basis = make_gpu_basis(gpu_ctx, sysm)
+    cd = to_dev(sysm.psi)
+    basis.gram(cd.data_ptr(), basis.ngw, sysm.nst)
+    gpu_ctx.sync()
+    got = from_dev(cd, basis.ngw, sysm.nst)
+    ref = oh.gram(sysm.psi, sysm.real)
+    basis.close()
+    assert rel_err(got, ref) <= 1e-11
+
+
+def test_exponential_complex_si8(gpu_ctx, si8c):
+    from oracle import hpsi as oh
+    ob, maps, sp = oracle_for(si8c)
+    basis = make_gpu_basis(gpu_ctx, si8c)
+    ham = make_gpu_ham(gpu_ctx, basis, si8c)
+    cd = to_dev(si8c.psi)
+    dt = 0.05
+    ham.exponential(cd.data_ptr(), basis.ngw, si8c.nst, dt)
+    gpu_ctx.sync()
+    got = from_dev(cd, basis.ngw, si8c.nst)
+    occ0 = np.zeros(si8c.nst)
+
+    def apply_h(w):
+        return oh.hpsi(maps, w, si8c.v, [sp], occ0, si8c.omega)[0]
+    ref = oh.exponential(apply_h, si8c.psi, dt)
+    ham.close()
+    basis.close()
+    assert rel_err(got, ref) <= FP64_TOL
+
+
+def test_vector_potential_complex_si8(gpu_ctx, si8c):
+    """Kinetic |k+G-A|^2 and projector phases at k+G-A (twnl kept fixed: the caller
+    re-tables it)."""
+    import torch
+    from oracle import hpsi as oh
+    ob, maps, sp = oracle_for(si8c)
+    basis = make_gpu_basis(gpu_ctx, si8c)
+    ham = make_gpu_ham(gpu_ctx, basis, si8c)
+    A = np.array([0.01, -0.02, 0.005])
+    ham.set_vector_potential(A)
+    cd = to_dev(si8c.psi)
+    hd = torch.zeros_like(cd)
+    ham.hpsi(cd.data_ptr(), basis.ngw, si8c.nst, si8c.occ, hd.data_ptr(), basis.ngw)
+    gpu_ctx.sync()
+    got = from_dev(hd, basis.ngw, si8c.nst)
+    ref = oh.hpsi(maps, si8c.psi, si8c.v, [sp], si8c.occ, si8c.omega,
+                  kpg2=oh.kpgpa2(ob, A), kpgx=ob.kpgx - A[None, :])[0]
+    ham.close()
+    basis.close()
+    assert rel_err(got, ref) <= FP64_TOL
