@@ -1,0 +1,238 @@
+#!/usr/bin/env python3
+"""Hψ throughput benchmark: band-applications/s of a full Hamiltonian application
+(non-local KB projectors + kinetic + local potential via the G<->r transforms) on diamond
+Si512 at Γ (a = 41.04 bohr, ecut 25 Ry: 72919 plane waves, 1024 bands, 140^3 grid,
+512 x 4 projectors), one MI355X.
+
+One step = one Hψ over all 1024 bands (qbh_hpsi with QBH_ALL), inputs resident in HBM.
+The projector table is generated on the first (warm-up) application and reused, as the
+ions do not move.  Prints one JSON line.
+"""
+import argparse
+import json
+import math
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+PEAK_HBM_GBS = 8000.0
+PEAK_FP64_TFLOPS = 78.6
+PEAK_FP32_TFLOPS = 157.3
+
+
+def parse():
+    p = argparse.ArgumentParser()
+    p.add_argument("--gpus", type=int, default=1)
+    p.add_argument("--steps", type=int, default=5)
+    p.add_argument("--warmup", type=int, default=2)
+    p.add_argument("--ncell", type=int, default=4, help="diamond cells per edge (4 = Si512)")
+    p.add_argument("--no-cpu-baseline", action="store_true")
+    p.add_argument("--cpu-sample-pairs", type=int, default=4)
+    p.add_argument("--no-fp32", action="store_true")
+    return p.parse_args()
+
+
+def kernel_bytes(sysm, nslots, prec_bytes):
+    """Algorithmic HBM bytes per launch of each transform kernel for a chunk of nslots
+    band pairs (complex element size prec_bytes; psi / hpsi stay FP64)."""
+    n0, n1, n2 = sysm.np
+    ngw, ncol, nlx = sysm.ngw, sysm.ncol, sysm.nlx
+    s = prec_bytes
+    t1 = ncol * n2 * s
+    t2 = n2 * nlx * n1 * s
+    psi_pair = 2 * ngw * 16
+    N = n0 * n1 * n2
+    return {
+        "zpass_bwd": nslots * (psi_pair + t1),
+        "ypass_bwd": nslots * (t1 + t2),
+        "xpass_vmul": nslots * (2 * t2) + N * 8,
+        "ypass_fwd": nslots * (t2 + t1),
+        "zpass_fwd": nslots * (t1 + psi_pair + 2 * psi_pair),
+    }
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print(f"bench.py runs on one GPU only; --gpus {args.gpus} is not supported "
+              "(the Hψ path shards by bands; see DESIGN.md)", file=sys.stderr)
+        sys.exit(2)
+    import torch
+    from qball_amd import Basis, Context, Hamiltonian, QBH_ALL, QBH_FP32
+    from qball_amd.workload import make_system
+
+    t_setup = time.time()
+    sysm = make_system(ncell=args.ncell, ecut_ry=25.0, seed=0)
+    dev = torch.device("cuda:0")
+    stream = torch.cuda.current_stream()
+    ctx = Context(0, stream.cuda_stream)
+    r = sysm.rods
+    basis = Basis(ctx, True, r["rod_h"], r["rod_k"], r["rod_lmin"], r["rod_size"], sysm.np, sysm.b)
+    sysm.ncol, sysm.nlx = basis.ncol, basis.nlx
+    ham = Hamiltonian(ctx, basis)
+    v = torch.from_numpy(sysm.v.copy()).to(dev)
+    ham.set_vloc(v.data_ptr())
+    ham.set_projectors([sysm.na], [4], sysm.lproj, sysm.wt, sysm.twnl.ravel(), sysm.tau.ravel(),
+                       sysm.omega)
+    ngw, nst = sysm.ngw, sysm.nst
+    psi = torch.from_numpy(np.ascontiguousarray(sysm.psi.T).view(np.float64).ravel().copy()).to(dev)
+    del sysm.psi
+    basis.gram(psi.data_ptr(), ngw, nst)       # orthonormalise (SlaterDet::gram on the GPU)
+    hpsi = torch.empty_like(psi)
+    ctx.sync()
+    t_setup = time.time() - t_setup
+
+    def step(flags):
+        ham.hpsi(psi.data_ptr(), ngw, nst, sysm.occ, hpsi.data_ptr(), ngw, flags=flags)
+
+    for _ in range(args.warmup):
+        step(QBH_ALL)
+    torch.cuda.synchronize()
+    ctx.timing_report()                        # discard warm-up totals
+    ctx.set_timing(True)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        step(QBH_ALL)
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    ctx.set_timing(False)
+    ktimes = ctx.timing_report()
+    dt = (t1 - t0) / args.steps
+    value = nst / dt
+
+    # energy-like check of this application: band sum of <psi|H|psi> (Γ: 2 Re - G=0 term)
+    def band_energy(h):
+        p = psi.view(nst, ngw, 2)
+        q = h.view(nst, ngw, 2)
+        d = 2.0 * (p * q).sum(dim=(1, 2)) - p[:, 0, 0] * q[:, 0, 0]
+        return float((torch.from_numpy(sysm.occ).to(dev) * d).sum())
+    e64 = band_energy(hpsi)
+
+    # roofline of the dominant kernel
+    nslots_total = (nst + 1) // 2
+    chunk_bytes = kernel_bytes(sysm, 1, 16)
+    flops_gemm = 2.0 * (2 * ngw) * nst * (4 * sysm.na)  # one GEMM: 2*M*N*K
+    per_kernel = {}
+    for name, (ms, cnt) in ktimes.items():
+        calls = cnt / args.steps
+        avg_ms = ms / cnt
+        entry = {"avg_ms": avg_ms, "launches_per_step": calls}
+        if name in chunk_bytes:
+            # launches per step = chunks; bytes scale with the slots in each chunk
+            b_step = kernel_bytes(sysm, nslots_total, 16)[name]
+            b_launch = b_step / calls
+            entry["bytes_per_launch"] = b_launch
+            entry["GBps"] = b_launch / (avg_ms * 1e-3) / 1e9
+        if name in ("fnl_gemm", "enl_gemm"):
+            entry["flops_per_launch"] = flops_gemm
+            entry["TFLOPs"] = flops_gemm / (avg_ms * 1e-3) / 1e12
+        per_kernel[name] = entry
+    dom = max(per_kernel, key=lambda k: per_kernel[k]["avg_ms"] * per_kernel[k]["launches_per_step"])
+    d = per_kernel[dom]
+    if "TFLOPs" in d:
+        roof = {"kernel": dom, "bound": "mfma", "achieved": d["TFLOPs"], "peak": PEAK_FP64_TFLOPS,
+                "unit": "TFLOP/s", "frac": d["TFLOPs"] / PEAK_FP64_TFLOPS, "traffic": None}
+    else:
+        roof = {"kernel": dom, "bound": "hbm", "achieved": d.get("GBps"), "peak": PEAK_HBM_GBS,
+                "unit": "GB/s", "frac": (d.get("GBps") or 0) / PEAK_HBM_GBS, "traffic": None}
+
+    # FP32-bulk form
+    fp32 = None
+    if not args.no_fp32:
+        h64 = hpsi.clone()
+        for _ in range(args.warmup):
+            step(QBH_ALL | QBH_FP32)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            step(QBH_ALL | QBH_FP32)
+        torch.cuda.synchronize()
+        dt32 = (time.perf_counter() - t0) / args.steps
+        e32 = band_energy(hpsi)
+        rel = float(torch.linalg.vector_norm(hpsi - h64) / torch.linalg.vector_norm(h64))
+        fp32 = {"value": nst / dt32, "ms_per_step": dt32 * 1e3, "speedup_vs_fp64": dt / dt32,
+                "hpsi_rel_err_vs_fp64": rel,
+                "band_energy_diff_per_atom_Ha": abs(e32 - e64) / sysm.na}
+        del h64
+
+    # CPU baseline: the oracle (numpy restatement) on a bounded sample, 1 thread
+    cpu = None
+    if not args.no_cpu_baseline:
+        cpu = cpu_baseline(sysm, args.cpu_sample_pairs, psi, ngw, nst)
+
+    out = {
+        "metric": "Hψ band-applications/s at Si512 Γ (FP64 plain form)",
+        "value": value,
+        "unit": "band-applications/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": dt * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic (seeded psi, smooth random v_loc, analytic s+p KB projectors)",
+        "config": {"workload": f"Si{sysm.na} diamond Γ, a={10.26 * args.ncell:.2f} bohr, ecut 25 Ry, "
+                               f"{ngw} PWs, {nst} bands, grid {sysm.np[0]}^3, {4 * sysm.na} KB projectors",
+                   "global_batch": nst, "parallelism": "single GPU"},
+        "roofline": roof,
+        "cpu_baseline": cpu,
+        "fp32_bulk": fp32,
+        "kernels": per_kernel,
+        "band_energy_Ha": e64,
+        "setup_s": t_setup,
+    }
+    print(json.dumps(out))
+    ham.close()
+    basis.close()
+    ctx.close()
+
+
+def cpu_baseline(sysm, npairs, psi_dev, ngw, nst):
+    """Oracle Hψ on a bounded sample: `npairs` band pairs through the local + kinetic path
+    and the non-local path (projector table built once), one thread; scaled to all bands
+    as t = t_anl + nst * t_per_band."""
+    from threadpoolctl import threadpool_limits
+    from oracle import hpsi as oh
+    from oracle.basis import Basis as OB, FFTMaps, UnitCell
+    ob = OB(UnitCell(sysm.a), sysm.ecut)
+    maps = FFTMaps(ob, *sysm.np)
+    nb = 2 * npairs
+    c = psi_dev[: 2 * ngw * nb].cpu().numpy().view(np.complex128).reshape(nb, ngw).T.copy()
+    sp = oh.Projectors(sysm.lproj, sysm.wt, sysm.twnl, sysm.tau)
+    with threadpool_limits(limits=1):
+        t0 = time.perf_counter()
+        anl = [oh.anl_block(sp, ob.kpgx, i, min(sp.na, i + 128)) for i in range(0, sp.na, 128)]
+        t_anl = time.perf_counter() - t0
+        del anl
+        cp = np.zeros_like(c)
+        t0 = time.perf_counter()
+        oh.kinetic_add(ob.kpg2, c, cp)
+        oh.rs_mul_add(maps, c, sysm.v, cp)
+        t_loc = time.perf_counter() - t0
+        # non-local GEMMs on the sample bands (table build timed above, excluded here)
+        t_nl = 0.0
+        for i in range(0, sp.na, 128):
+            a = oh.anl_block(sp, ob.kpgx, i, min(sp.na, i + 128))
+            t1 = time.perf_counter()
+            f = 2.0 * (a.real.T @ c.real + a.imag.T @ c.imag)
+            cp += a.real @ f + 1j * (a.imag @ f)
+            t_nl += time.perf_counter() - t1
+    per_band = (t_loc + t_nl) / nb
+    t_full = t_anl + nst * per_band
+    return {"value": nst / t_full, "unit": "band-applications/s", "cores": 1, "kind": "port",
+            "sample": f"{nb} of {nst} bands through oracle local+kinetic+non-local Hψ "
+                      f"(numpy/pocketfft, 1 thread) plus one projector-table build "
+                      f"({t_anl:.1f} s); full-Hψ time extrapolated as t_table + nst*t_band",
+            "t_band_s": per_band, "t_table_s": t_anl}
+
+
+if __name__ == "__main__":
+    main()

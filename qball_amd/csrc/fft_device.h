@@ -193,6 +193,82 @@ __device__ __forceinline__ void stockham_pass(cplx<T>* buf, int nlines, int ld, 
   __syncthreads();
 }
 
+// ------------------------------------------------------------------------------------
+// Wavefront-local variants: the 64 lanes of one wavefront own `nlines` LDS lines and
+// transform them with no workgroup barrier.  LDS operations of one wavefront complete
+// in issue order, so a release/acquire fence at wavefront scope (which also keeps the
+// compiler from moving LDS accesses across it) is the only synchronisation needed
+// between the read and the write half of a pass.
+__device__ __forceinline__ void wave_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// EPL bounds the elements per lane: nlines * n <= EPL * 64.
+template <int R, int DIR, int EPL, typename T>
+__device__ __forceinline__ void stockham_pass_w(cplx<T>* buf, int nlines, int ld, int n, int Ns,
+                                                const cplx<T>* tw, int lane) {
+  constexpr int MAXIT = (EPL + R - 1) / R;
+  const int nb = n / R;
+  const int total = nlines * nb;
+  const int step = n / (Ns * R);
+  cplx<T> v[MAXIT][R];
+  int dst[MAXIT];
+#pragma unroll
+  for (int it = 0; it < MAXIT; ++it) {
+    const int w = lane + it * 64;
+    dst[it] = -1;
+    if (w < total) {
+      const int line = w / nb;
+      const int j = w - line * nb;
+      const int k = j % Ns;
+      const cplx<T>* base = buf + line * ld;
+#pragma unroll
+      for (int r = 0; r < R; ++r) v[it][r] = base[j + r * nb];
+#pragma unroll
+      for (int r = 1; r < R; ++r) {
+        cplx<T> t = tw[k * r * step];
+        if (DIR < 0) t.y = -t.y;
+        v[it][r] = cmul(v[it][r], t);
+      }
+      dft<R, DIR>(v[it]);
+      dst[it] = line * ld + (j / Ns) * Ns * R + k;
+    }
+  }
+  wave_sync();
+#pragma unroll
+  for (int it = 0; it < MAXIT; ++it) {
+    if (dst[it] >= 0) {
+#pragma unroll
+      for (int r = 0; r < R; ++r) buf[dst[it] + r * Ns] = v[it][r];
+    }
+  }
+  wave_sync();
+}
+
+// Wavefront-local transform of `nlines` lines at `buf` (this wavefront's lines only).
+// Caller: data written by this wavefront (or made visible by a barrier) before the call.
+template <int DIR, int EPL, typename T>
+__device__ __forceinline__ void fft_lines_w(cplx<T>* buf, int nlines, int ld, const AxisPlan& p,
+                                            const cplx<T>* tw, int lane) {
+  int Ns = 1;
+  for (int ip = 0; ip < p.npass; ++ip) {
+    const int R = p.radix[ip];
+    switch (R) {
+      case 2: stockham_pass_w<2, DIR, EPL>(buf, nlines, ld, p.n, Ns, tw, lane); break;
+      case 3: stockham_pass_w<3, DIR, EPL>(buf, nlines, ld, p.n, Ns, tw, lane); break;
+      case 4: stockham_pass_w<4, DIR, EPL>(buf, nlines, ld, p.n, Ns, tw, lane); break;
+      case 5: stockham_pass_w<5, DIR, EPL>(buf, nlines, ld, p.n, Ns, tw, lane); break;
+      case 7: stockham_pass_w<7, DIR, EPL>(buf, nlines, ld, p.n, Ns, tw, lane); break;
+      case 8: stockham_pass_w<8, DIR, EPL>(buf, nlines, ld, p.n, Ns, tw, lane); break;
+      case 11: stockham_pass_w<11, DIR, EPL>(buf, nlines, ld, p.n, Ns, tw, lane); break;
+      default: break;
+    }
+    Ns *= R;
+  }
+}
+
 // Full transform of `nlines` LDS lines.  Caller must __syncthreads() before (data in
 // place); returns after a barrier (result visible to all threads).
 template <int DIR, int EPT, typename T>

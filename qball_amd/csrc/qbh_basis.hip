@@ -299,13 +299,16 @@ extern "C" int qbh_basis_create(qbh_ctx ctx, int real, int nrods, const int* rod
   g.nlx = (int)line_x.size();
 
   // z-pass groups: consecutive columns in creation order (the +G/-G images of a rod
-  // are adjacent), at most kZL lines and at most 4096/np2 lines per group
-  const int maxl = std::min(kZL, 4096 / np2);
+  // are adjacent), at most kZL lines per group (4 per wavefront)
+  const int maxl = kZL;
   std::vector<int> grp_start{0}, ent_col, ent_rod, ent_kind;
   int inl = 0;
+  // Γ: rod(0,0) is followed by an empty line (kind 4) so that every +G/-G pair starts
+  // at an even line and never straddles the 4-line share of a wavefront
   for (int r = 0, ci = 0; r < nrods; ++r) {
     const int nc = (real && r > 0) ? 2 : 1;
-    if (inl + nc > maxl) {
+    const int nl = real ? 2 : 1;
+    if (inl + nl > maxl) {
       grp_start.push_back((int)ent_col.size());
       inl = 0;
     }
@@ -314,7 +317,12 @@ extern "C" int qbh_basis_create(qbh_ctx ctx, int real, int nrods, const int* rod
       ent_rod.push_back(r);
       ent_kind.push_back(cols[ci].kind);
     }
-    inl += nc;
+    if (real && r == 0) {
+      ent_col.push_back(-1);
+      ent_rod.push_back(0);
+      ent_kind.push_back(4);
+    }
+    inl += nl;
   }
   grp_start.push_back((int)ent_col.size());
   g.ngroups = (int)grp_start.size() - 1;

@@ -8,6 +8,11 @@
 // columns only, a y pass over the x-lines that hold sphere points, and an x pass that
 // owns whole real-space rows, so v(r)*psi(r) and |psi(r)|^2 are applied while a row is
 // in LDS and the real-space grid never goes to HBM.
+//
+// Work split: every workgroup has 4 wavefronts; each wavefront owns LPW lines of LDS and
+// transforms them on its own (fft_lines_w: no workgroup barrier inside a transform).
+// Global reads and writes are arranged so that a wavefront touches contiguous segments
+// of >= 64 bytes.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -18,13 +23,21 @@
 namespace qbh {
 
 constexpr int NTHR = 256;  // threads per workgroup, every pipeline kernel
-constexpr int EPT = 16;    // lines * n <= EPT * NTHR in every LDS transform
-constexpr int ZL = 16;     // max lines (columns) per z-pass group
-constexpr int YB = 16;     // real-space rows per x-pass workgroup
+constexpr int NWAVE = 4;
+constexpr int ZLPW = 4;    // z-pass lines per wavefront (16 per group)
 
-template <typename T> struct ZBlk;
-template <> struct ZBlk<double> { static constexpr int ZB = 8, LB = 2; };   // 128 B segments
-template <> struct ZBlk<float> { static constexpr int ZB = 16, LB = 1; };   // 128 B segments
+// per precision: lines per wavefront in the y and x passes, y-pass plane block
+template <typename T> struct Cfg;
+template <> struct Cfg<double> {
+  static constexpr int LPW = 4;   // 4 lines x 16 B x ... : 64 B row segments in the x pass
+  static constexpr int ZB = 8;    // y pass: 8 planes x 16 B = 128 B column segments
+  static constexpr int LB = 2;    // y pass: x-lines per workgroup (ZB*LB = 4*LPW)
+};
+template <> struct Cfg<float> {
+  static constexpr int LPW = 8;
+  static constexpr int ZB = 16;
+  static constexpr int LB = 2;
+};
 
 template <typename T>
 __device__ __forceinline__ const cplx<T>* twtab(const GridDev& g, int axis);
@@ -39,25 +52,32 @@ __device__ __forceinline__ const cplx<float>* twtab<float>(const GridDev& g, int
 
 __device__ __forceinline__ int wrap(int l, int n) { return l < 0 ? l + n : l; }
 
+// elements per lane of a wavefront owning `lpw` lines of length <= NMAX
+template <int LPW, int NMAX> struct Epl { static constexpr int v = (LPW * NMAX + 63) / 64; };
+
 // ------------------------------------------------------------------------------------
 // z pass, backward: gather sphere coefficients of one slot into its columns, transform
-// along z (exp(+i)), store T1[slot][col][z].
-template <typename T, int MODE>
+// along z (exp(+i)), store T1[slot][col][z].  Line e of a group belongs to wavefront e/4.
+template <typename T, int MODE, int NMAX>
 __global__ void __launch_bounds__(NTHR)
 k_zpass_bwd(GridDev g, const double* __restrict__ psi, long ld, int nst, int band0,
             cplx<T>* __restrict__ t1) {
+  constexpr int EPL = Epl<ZLPW, NMAX>::v;
   extern __shared__ __align__(16) unsigned char smem_raw[];
   const int np2 = g.np2;
   cplx<T>* tw = reinterpret_cast<cplx<T>*>(smem_raw);
-  cplx<T>* buf = tw + np2;
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
   const int grp = blockIdx.x;
   const int slot = blockIdx.y;
-  const int e0 = g.grp_start[grp];
-  const int nl = g.grp_start[grp + 1] - e0;
   const cplx<T>* twg = twtab<T>(g, 2);
   for (int i = threadIdx.x; i < np2; i += NTHR) tw[i] = twg[i];
-  for (int i = threadIdx.x; i < nl * np2; i += NTHR) buf[i] = {T(0), T(0)};
   __syncthreads();
+  const int e0 = g.grp_start[grp] + wave * ZLPW;
+  const int nl = min(ZLPW, g.grp_start[grp + 1] - e0);
+  if (nl <= 0) return;
+  cplx<T>* buf = tw + np2 + wave * ZLPW * np2;
+  for (int i = lane; i < nl * np2; i += 64) buf[i] = {T(0), T(0)};
+  wave_sync();
 
   int n1, n2;
   if (MODE == SLOT_PAIR) {
@@ -71,62 +91,68 @@ k_zpass_bwd(GridDev g, const double* __restrict__ psi, long ld, int nst, int ban
   const double* p1 = psi + 2 * (long)n1 * ld;
   const double* p2 = psi + 2 * (long)(has2 ? n2 : n1) * ld;
 
-  // 16 threads per line
-  const int le = threadIdx.x >> 4;
-  const int sub = threadIdx.x & 15;
+  // 16 lanes per line
+  const int le = lane >> 4;
+  const int sub = lane & 15;
   if (le < nl) {
     const int e = e0 + le;
-    const int rod = g.ent_rod[e];
     const int kind = g.ent_kind[e];
-    const int first = g.rod_first[rod];
-    const int lmin = g.rod_lmin[rod];
-    const int size = g.rod_size[rod];
-    cplx<T>* line = buf + le * np2;
-    for (int i = sub; i < size; i += 16) {
-      const int l = lmin + i;
-      const double a = p1[2 * (first + i)];
-      const double b = p1[2 * (first + i) + 1];
-      double c = 0.0, d = 0.0;
-      if (has2) {
-        c = p2[2 * (first + i)];
-        d = p2[2 * (first + i) + 1];
-      }
-      double pr, pi, mr, mi;  // +G and -G values
-      if (MODE == SLOT_PAIR) {
-        pr = a - d; pi = b + c; mr = a + d; mi = c - b;
-      } else {
-        pr = a; pi = b; mr = a; mi = -b;
-      }
-      if (kind == 0) {
-        if (i > 0) line[wrap(l, np2)] = {T(pr), T(pi)};
-        line[wrap(-l, np2)] = {T(mr), T(mi)};
-      } else if (kind == 1 || kind == 3) {
-        line[wrap(l, np2)] = {T(pr), T(pi)};
-      } else {
-        line[wrap(-l, np2)] = {T(mr), T(mi)};
+    if (kind != 4) {
+      const int rod = g.ent_rod[e];
+      const int first = g.rod_first[rod];
+      const int lmin = g.rod_lmin[rod];
+      const int size = g.rod_size[rod];
+      cplx<T>* line = buf + le * np2;
+      for (int i = sub; i < size; i += 16) {
+        const int l = lmin + i;
+        const double a = p1[2 * (first + i)];
+        const double b = p1[2 * (first + i) + 1];
+        double c = 0.0, d = 0.0;
+        if (has2) {
+          c = p2[2 * (first + i)];
+          d = p2[2 * (first + i) + 1];
+        }
+        double pr, pi, mr, mi;  // +G and -G values
+        if (MODE == SLOT_PAIR) {
+          pr = a - d; pi = b + c; mr = a + d; mi = c - b;
+        } else {
+          pr = a; pi = b; mr = a; mi = -b;
+        }
+        if (kind == 0) {
+          if (i > 0) line[wrap(l, np2)] = {T(pr), T(pi)};
+          line[wrap(-l, np2)] = {T(mr), T(mi)};
+        } else if (kind == 1 || kind == 3) {
+          line[wrap(l, np2)] = {T(pr), T(pi)};
+        } else {
+          line[wrap(-l, np2)] = {T(mr), T(mi)};
+        }
       }
     }
   }
-  __syncthreads();
-  fft_lines<+1, EPT>(buf, nl, np2, g.ax[2], tw);
-  for (int i = threadIdx.x; i < nl * np2; i += NTHR) {
-    const int ll = i / np2;
-    const int z = i - ll * np2;
+  wave_sync();
+  fft_lines_w<+1, EPL>(buf, nl, np2, g.ax[2], tw, lane);
+  for (int ll = 0; ll < nl; ++ll) {
     const int col = g.ent_col[e0 + ll];
-    t1[((long)slot * g.ncol + col) * np2 + z] = buf[i];
+    if (col < 0) continue;
+    cplx<T>* dst = t1 + ((long)slot * g.ncol + col) * np2;
+    for (int z = lane; z < np2; z += 64) dst[z] = buf[ll * np2 + z];
   }
 }
 
 // ------------------------------------------------------------------------------------
 // y pass, backward: T1 columns of ZB planes x LB x-lines -> y transform -> T2 lines.
-template <typename T>
+// The workgroup gathers its columns cooperatively (ZB-element segments), then each
+// wavefront transforms and stores its own LPW lines.
+template <typename T, int NMAX>
 __global__ void __launch_bounds__(NTHR)
 k_ypass_bwd(GridDev g, const cplx<T>* __restrict__ t1, cplx<T>* __restrict__ t2) {
-  constexpr int ZB = ZBlk<T>::ZB, LB = ZBlk<T>::LB;
+  constexpr int ZB = Cfg<T>::ZB, LB = Cfg<T>::LB, LPW = Cfg<T>::LPW;
+  constexpr int EPL = Epl<LPW, NMAX>::v;
   extern __shared__ __align__(16) unsigned char smem_raw[];
   const int np1 = g.np1, np2 = g.np2, nlx = g.nlx;
   cplx<T>* tw = reinterpret_cast<cplx<T>*>(smem_raw);
   cplx<T>* buf = tw + np1;
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
   const int nzb = (np2 + ZB - 1) / ZB;
   const int zb = blockIdx.x % nzb;
   const int lb = blockIdx.x / nzb;
@@ -144,32 +170,36 @@ k_ypass_bwd(GridDev g, const cplx<T>* __restrict__ t1, cplx<T>* __restrict__ t2)
     const int c = c0 + f / ZB;
     const int z = z0 + dz;
     if (z < np2) {
-      // line of column c: find by scanning the (<= LB) line starts
       int dl = 0;
       while (lx0 + dl + 1 < lx1 && c >= g.line_start[lx0 + dl + 1]) ++dl;
       buf[(dz + ZB * dl) * np1 + g.col_y[c]] = t1[((long)slot * g.ncol + c) * np2 + z];
     }
   }
   __syncthreads();
-  fft_lines<+1, EPT>(buf, ZB * LB, np1, g.ax[1], tw);
-  for (int i = threadIdx.x; i < ZB * LB * np1; i += NTHR) {
-    const int li = i / np1;
-    const int y = i - li * np1;
+  // line li = dz + ZB*dl; wavefront w owns lines [w*LPW, (w+1)*LPW)
+  cplx<T>* wbuf = buf + wave * LPW * np1;
+  fft_lines_w<+1, EPL>(wbuf, LPW, np1, g.ax[1], tw, lane);
+  for (int ll = 0; ll < LPW; ++ll) {
+    const int li = wave * LPW + ll;
     const int dz = li % ZB, dl = li / ZB;
     const int z = z0 + dz, lx = lx0 + dl;
-    if (z < np2 && lx < lx1) t2[(((long)slot * np2 + z) * nlx + lx) * np1 + y] = buf[i];
+    if (z >= np2 || lx >= lx1) continue;
+    cplx<T>* dst = t2 + (((long)slot * np2 + z) * nlx + lx) * np1;
+    for (int y = lane; y < np1; y += 64) dst[y] = wbuf[ll * np1 + y];
   }
 }
 
 // y pass, forward: T2 lines -> y transform (exp(-i)) -> sphere columns of T1.
-template <typename T>
+template <typename T, int NMAX>
 __global__ void __launch_bounds__(NTHR)
 k_ypass_fwd(GridDev g, const cplx<T>* __restrict__ t2, cplx<T>* __restrict__ t1) {
-  constexpr int ZB = ZBlk<T>::ZB, LB = ZBlk<T>::LB;
+  constexpr int ZB = Cfg<T>::ZB, LB = Cfg<T>::LB, LPW = Cfg<T>::LPW;
+  constexpr int EPL = Epl<LPW, NMAX>::v;
   extern __shared__ __align__(16) unsigned char smem_raw[];
   const int np1 = g.np1, np2 = g.np2, nlx = g.nlx;
   cplx<T>* tw = reinterpret_cast<cplx<T>*>(smem_raw);
   cplx<T>* buf = tw + np1;
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
   const int nzb = (np2 + ZB - 1) / ZB;
   const int zb = blockIdx.x % nzb;
   const int lb = blockIdx.x / nzb;
@@ -178,17 +208,19 @@ k_ypass_fwd(GridDev g, const cplx<T>* __restrict__ t2, cplx<T>* __restrict__ t1)
   const int lx1 = min(lx0 + LB, nlx);
   const cplx<T>* twg = twtab<T>(g, 1);
   for (int i = threadIdx.x; i < np1; i += NTHR) tw[i] = twg[i];
-  for (int i = threadIdx.x; i < ZB * LB * np1; i += NTHR) {
-    const int li = i / np1;
-    const int y = i - li * np1;
+  __syncthreads();
+  cplx<T>* wbuf = buf + wave * LPW * np1;
+  for (int ll = 0; ll < LPW; ++ll) {
+    const int li = wave * LPW + ll;
     const int dz = li % ZB, dl = li / ZB;
     const int z = z0 + dz, lx = lx0 + dl;
-    cplx<T> v = {T(0), T(0)};
-    if (z < np2 && lx < lx1) v = t2[(((long)slot * np2 + z) * nlx + lx) * np1 + y];
-    buf[i] = v;
+    const bool ok = z < np2 && lx < lx1;
+    const cplx<T>* src = t2 + (((long)slot * np2 + (ok ? z : 0)) * nlx + (ok ? lx : 0)) * np1;
+    for (int y = lane; y < np1; y += 64) wbuf[ll * np1 + y] = ok ? src[y] : cplx<T>{T(0), T(0)};
   }
+  wave_sync();
+  fft_lines_w<-1, EPL>(wbuf, LPW, np1, g.ax[1], tw, lane);
   __syncthreads();
-  fft_lines<-1, EPT>(buf, ZB * LB, np1, g.ax[1], tw);
   const int c0 = g.line_start[lx0], c1 = g.line_start[lx1];
   const int nwork = (c1 - c0) * ZB;
   for (int f = threadIdx.x; f < nwork; f += NTHR) {
@@ -204,214 +236,200 @@ k_ypass_fwd(GridDev g, const cplx<T>* __restrict__ t2, cplx<T>* __restrict__ t1)
 }
 
 // ------------------------------------------------------------------------------------
-// x pass.  One workgroup owns YB real-space rows (z fixed, y0..y0+YB) and loops over
-// the slots [s0, s1).
+// x pass.  Wavefront w of a workgroup owns LPW consecutive real-space rows (z fixed,
+// y = y0 + w*LPW ...) and loops over the slots [s0, s1) with no workgroup barrier; the
+// next slot's sphere lines are loaded into registers while the current slot transforms.
 enum XOp { X_VMUL = 0, X_STORE = 1, X_LOAD = 2, X_LOAD_REAL = 3, X_DENSITY = 4 };
 
-template <typename T, int OP>
+template <typename T, int OP, int NMAX>
 __global__ void __launch_bounds__(NTHR)
 k_xpass(GridDev g, cplx<T>* __restrict__ t2, int nslots, int slots_per_wg,
         const double* __restrict__ v,      // VMUL: potential; LOAD*: input grid
         double* __restrict__ out,          // STORE: complex grid; DENSITY: rho
         const double* __restrict__ dens,   // DENSITY: per slot (fac1, fac2, beta1, beta2)
-        double scale) {
+        double scale, int cplx_density) {
+  constexpr int LPW = Cfg<T>::LPW;
+  constexpr int YB = LPW * NWAVE;
+  constexpr int EPL = Epl<LPW, NMAX>::v;          // row elements per lane
+  constexpr int GPL = EPL;  // sphere-line elements per lane (nlx <= np0)
   extern __shared__ __align__(16) unsigned char smem_raw[];
   const int np0 = g.np0, np1 = g.np1, np2 = g.np2, nlx = g.nlx;
   cplx<T>* tw = reinterpret_cast<cplx<T>*>(smem_raw);
-  cplx<T>* buf = tw + np0;
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
   const int nyb = (np1 + YB - 1) / YB;
   const int z = blockIdx.x / nyb;
-  const int y0 = (blockIdx.x % nyb) * YB;
-  const int nyl = min(YB, np1 - y0);  // rows in this block
+  const int y0 = (blockIdx.x % nyb) * YB + wave * LPW;  // this wavefront's first row
+  const int nyl = min(LPW, np1 - y0);                   // rows of this wavefront
   const int s0 = blockIdx.y * slots_per_wg;
   const int s1 = min(nslots, s0 + slots_per_wg);
   const cplx<T>* twg = twtab<T>(g, 0);
   for (int i = threadIdx.x; i < np0; i += NTHR) tw[i] = twg[i];
+  __syncthreads();
+  if (nyl <= 0) return;
+  cplx<T>* buf = tw + np0 + wave * LPW * np0;
   const int nel = nyl * np0;
-  const long grow0 = ((long)z * np1 + y0) * np0;  // grid offset of the block's first row
+  const long grow0 = ((long)z * np1 + y0) * np0;  // grid offset of the wavefront's first row
+  const long N = (long)np0 * np1 * np2;
+  const int ng = nlx * LPW;                       // sphere-line elements of the rows
 
-  T vr[EPT];
-  double acc[EPT];
+  T vr[EPL];
+  double acc[EPL];
   if (OP == X_VMUL) {
 #pragma unroll
-    for (int k = 0; k < EPT; ++k) {
-      const int e = threadIdx.x + k * NTHR;
+    for (int k = 0; k < EPL; ++k) {
+      const int e = lane + k * 64;
       vr[k] = e < nel ? T(v[grow0 + e]) : T(0);
     }
   }
   if (OP == X_DENSITY) {
 #pragma unroll
-    for (int k = 0; k < EPT; ++k) acc[k] = 0.0;
+    for (int k = 0; k < EPL; ++k) acc[k] = 0.0;
   }
 
+  // sphere-line element q = lane + k*64: row dy = q % LPW, line lx = q / LPW
+  cplx<T> pf[GPL];
+  auto prefetch = [&](int s) {
+#pragma unroll
+    for (int k = 0; k < GPL; ++k) {
+      const int q = lane + k * 64;
+      const int dy = q % LPW, lx = q / LPW;
+      pf[k] = (q < ng && dy < nyl) ? t2[(((long)s * np2 + z) * nlx + lx) * np1 + y0 + dy]
+                                   : cplx<T>{T(0), T(0)};
+    }
+  };
+  const bool sparse_in = !(OP == X_LOAD || OP == X_LOAD_REAL);
+  if (sparse_in && s0 < s1) prefetch(s0);
+
   for (int s = s0; s < s1; ++s) {
-    __syncthreads();
-    if (OP == X_LOAD || OP == X_LOAD_REAL) {
-      for (int e = threadIdx.x; e < YB * np0; e += NTHR) {
+    if (sparse_in) {
+      for (int i = lane; i < LPW * np0; i += 64) buf[i] = {T(0), T(0)};
+      wave_sync();
+#pragma unroll
+      for (int k = 0; k < GPL; ++k) {
+        const int q = lane + k * 64;
+        if (q < ng) buf[(q % LPW) * np0 + g.line_x[q / LPW]] = pf[k];
+      }
+      wave_sync();
+      if (s + 1 < s1) prefetch(s + 1);
+      fft_lines_w<+1, EPL>(buf, LPW, np0, g.ax[0], tw, lane);
+    } else {
+      for (int e = lane; e < LPW * np0; e += 64) {
         cplx<T> val = {T(0), T(0)};
         if (e < nel) {
           if (OP == X_LOAD) {
-            const double* fp = v + 2 * ((long)s * np0 * np1 * np2 + grow0 + e);
+            const double* fp = v + 2 * ((long)s * N + grow0 + e);
             val = {T(fp[0]), T(fp[1])};
           } else {
-            val = {T(scale * v[(long)s * np0 * np1 * np2 + grow0 + e]), T(0)};
+            val = {T(scale * v[(long)s * N + grow0 + e]), T(0)};
           }
         }
         buf[e] = val;
       }
-      __syncthreads();
-    } else {
-      for (int e = threadIdx.x; e < YB * np0; e += NTHR) buf[e] = {T(0), T(0)};
-      __syncthreads();
-      const int nwork = nlx * YB;
-      for (int f = threadIdx.x; f < nwork; f += NTHR) {
-        const int dy = f % YB;
-        const int lx = f / YB;
-        if (dy < nyl)
-          buf[dy * np0 + g.line_x[lx]] =
-              t2[(((long)s * np2 + z) * nlx + lx) * np1 + y0 + dy];
-      }
-      __syncthreads();
-      fft_lines<+1, EPT>(buf, YB, np0, g.ax[0], tw);
+      wave_sync();
     }
 
     if (OP == X_VMUL) {
 #pragma unroll
-      for (int k = 0; k < EPT; ++k) {
-        const int e = threadIdx.x + k * NTHR;
+      for (int k = 0; k < EPL; ++k) {
+        const int e = lane + k * 64;
         if (e < nel) {
-          cplx<T> p = buf[e];
+          const cplx<T> p = buf[e];
           buf[e] = {vr[k] * p.x, vr[k] * p.y};
         }
       }
-      __syncthreads();
+      wave_sync();
     } else if (OP == X_STORE) {
-      for (int e = threadIdx.x; e < nel; e += NTHR) {
-        double* fp = out + 2 * ((long)s * np0 * np1 * np2 + grow0 + e);
+      for (int e = lane; e < nel; e += 64) {
+        double* fp = out + 2 * ((long)s * N + grow0 + e);
         fp[0] = double(buf[e].x);
         fp[1] = double(buf[e].y);
       }
+      wave_sync();
       continue;
     } else if (OP == X_DENSITY) {
-      const double f1 = dens[4 * s + 0], f2 = dens[4 * s + 1];
-      const double b1 = dens[4 * s + 2], b2 = dens[4 * s + 3];
+      if (cplx_density) {
+        const double f = dens[s];
 #pragma unroll
-      for (int k = 0; k < EPT; ++k) {
-        const int e = threadIdx.x + k * NTHR;
-        if (e < nel) {
-          const cplx<T> p = buf[e];
-          // f = psi1 + i psi2 with psi_j = X_j + conj(c_j(G=0)): Re psi1 = Re f - beta2,
-          // Re psi2 = Im f + beta1, Im psi_j = -beta_j
-          if (f1 > 0.0) {
-            const double r1 = double(p.x) - b2;
-            acc[k] += f1 * (r1 * r1 + b1 * b1);
+        for (int k = 0; k < EPL; ++k) {
+          const int e = lane + k * 64;
+          if (e < nel && f > 0.0) {
+            const double re = double(buf[e].x), im = double(buf[e].y);
+            acc[k] += f * (re * re + im * im);
           }
-          if (f2 > 0.0) {
-            const double r2 = double(p.y) + b1;
-            acc[k] += f2 * (r2 * r2 + b2 * b2);
+        }
+      } else {
+        const double f1 = dens[4 * s + 0], f2 = dens[4 * s + 1];
+        const double b1 = dens[4 * s + 2], b2 = dens[4 * s + 3];
+#pragma unroll
+        for (int k = 0; k < EPL; ++k) {
+          const int e = lane + k * 64;
+          if (e < nel) {
+            const cplx<T> p = buf[e];
+            // f = psi1 + i psi2 with psi_j = X_j + conj(c_j(G=0)): Re psi1 = Re f - beta2,
+            // Re psi2 = Im f + beta1, Im psi_j = -beta_j
+            if (f1 > 0.0) {
+              const double r1 = double(p.x) - b2;
+              acc[k] += f1 * (r1 * r1 + b1 * b1);
+            }
+            if (f2 > 0.0) {
+              const double r2 = double(p.y) + b1;
+              acc[k] += f2 * (r2 * r2 + b2 * b2);
+            }
           }
         }
       }
+      wave_sync();
       continue;
     }
 
-    if (OP == X_VMUL || OP == X_LOAD || OP == X_LOAD_REAL) {
-      if (OP != X_VMUL) __syncthreads();
-      fft_lines<-1, EPT>(buf, YB, np0, g.ax[0], tw);
-      const int nwork = nlx * YB;
-      for (int f = threadIdx.x; f < nwork; f += NTHR) {
-        const int dy = f % YB;
-        const int lx = f / YB;
-        if (dy < nyl)
-          t2[(((long)s * np2 + z) * nlx + lx) * np1 + y0 + dy] = buf[dy * np0 + g.line_x[lx]];
-      }
+    // VMUL / LOAD*: forward x transform and store of the sphere lines
+    fft_lines_w<-1, EPL>(buf, LPW, np0, g.ax[0], tw, lane);
+    for (int q = lane; q < ng; q += 64) {
+      const int dy = q % LPW, lx = q / LPW;
+      if (dy < nyl)
+        t2[(((long)s * np2 + z) * nlx + lx) * np1 + y0 + dy] = buf[dy * np0 + g.line_x[lx]];
     }
+    wave_sync();
   }
   if (OP == X_DENSITY) {
 #pragma unroll
-    for (int k = 0; k < EPT; ++k) {
-      const int e = threadIdx.x + k * NTHR;
+    for (int k = 0; k < EPL; ++k) {
+      const int e = lane + k * 64;
       if (e < nel) out[grow0 + e] += acc[k];
     }
-  }
-}
-
-// Complex-basis density: one band per slot, rho += fac * |f|^2.
-template <typename T>
-__global__ void __launch_bounds__(NTHR)
-k_xpass_density_complex(GridDev g, const cplx<T>* __restrict__ t2, int nslots,
-                        double* __restrict__ rho, const double* __restrict__ facs) {
-  extern __shared__ __align__(16) unsigned char smem_raw[];
-  const int np0 = g.np0, np1 = g.np1, np2 = g.np2, nlx = g.nlx;
-  cplx<T>* tw = reinterpret_cast<cplx<T>*>(smem_raw);
-  cplx<T>* buf = tw + np0;
-  const int nyb = (np1 + YB - 1) / YB;
-  const int z = blockIdx.x / nyb;
-  const int y0 = (blockIdx.x % nyb) * YB;
-  const int nyl = min(YB, np1 - y0);
-  const cplx<T>* twg = twtab<T>(g, 0);
-  for (int i = threadIdx.x; i < np0; i += NTHR) tw[i] = twg[i];
-  const int nel = nyl * np0;
-  const long grow0 = ((long)z * np1 + y0) * np0;
-  double acc[EPT];
-#pragma unroll
-  for (int k = 0; k < EPT; ++k) acc[k] = 0.0;
-  for (int s = 0; s < nslots; ++s) {
-    __syncthreads();
-    for (int e = threadIdx.x; e < YB * np0; e += NTHR) buf[e] = {T(0), T(0)};
-    __syncthreads();
-    for (int f = threadIdx.x; f < nlx * YB; f += NTHR) {
-      const int dy = f % YB;
-      const int lx = f / YB;
-      if (dy < nyl)
-        buf[dy * np0 + g.line_x[lx]] = t2[(((long)s * np2 + z) * nlx + lx) * np1 + y0 + dy];
-    }
-    __syncthreads();
-    fft_lines<+1, EPT>(buf, YB, np0, g.ax[0], tw);
-    const double fac = facs[s];
-    if (fac > 0.0) {
-#pragma unroll
-      for (int k = 0; k < EPT; ++k) {
-        const int e = threadIdx.x + k * NTHR;
-        if (e < nel) {
-          const double re = double(buf[e].x), im = double(buf[e].y);
-          acc[k] += fac * (re * re + im * im);
-        }
-      }
-    }
-  }
-#pragma unroll
-  for (int k = 0; k < EPT; ++k) {
-    const int e = threadIdx.x + k * NTHR;
-    if (e < nel) rho[grow0 + e] += acc[k];
   }
 }
 
 // ------------------------------------------------------------------------------------
 // z pass, forward: T1 columns -> z transform (exp(-i)) -> x 1/N -> unpack -> output.
 // out[n][G] = value (store) or out[n][G] = (out + kin[G]*psi[n][G]) + value.
-template <typename T, int MODE>
+template <typename T, int MODE, int NMAX>
 __global__ void __launch_bounds__(NTHR)
 k_zpass_fwd(GridDev g, const cplx<T>* __restrict__ t1, const double* __restrict__ psi, long ld,
             int nst, int band0, double* __restrict__ out, long ldo,
             const double* __restrict__ kin, int store) {
+  constexpr int EPL = Epl<ZLPW, NMAX>::v;
   extern __shared__ __align__(16) unsigned char smem_raw[];
   const int np2 = g.np2;
   cplx<T>* tw = reinterpret_cast<cplx<T>*>(smem_raw);
-  cplx<T>* buf = tw + np2;
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
   const int grp = blockIdx.x;
   const int slot = blockIdx.y;
-  const int e0 = g.grp_start[grp];
-  const int nl = g.grp_start[grp + 1] - e0;
   const cplx<T>* twg = twtab<T>(g, 2);
   for (int i = threadIdx.x; i < np2; i += NTHR) tw[i] = twg[i];
-  for (int i = threadIdx.x; i < nl * np2; i += NTHR) {
-    const int ll = i / np2;
-    const int z = i - ll * np2;
-    buf[i] = t1[((long)slot * g.ncol + g.ent_col[e0 + ll]) * np2 + z];
-  }
   __syncthreads();
-  fft_lines<-1, EPT>(buf, nl, np2, g.ax[2], tw);
+  const int e0 = g.grp_start[grp] + wave * ZLPW;
+  const int nl = min(ZLPW, g.grp_start[grp + 1] - e0);
+  if (nl <= 0) return;
+  cplx<T>* buf = tw + np2 + wave * ZLPW * np2;
+  for (int ll = 0; ll < nl; ++ll) {
+    const int col = g.ent_col[e0 + ll];
+    const cplx<T>* src = t1 + ((long)slot * g.ncol + (col < 0 ? 0 : col)) * np2;
+    for (int z = lane; z < np2; z += 64)
+      buf[ll * np2 + z] = col < 0 ? cplx<T>{T(0), T(0)} : src[z];
+  }
+  wave_sync();
+  fft_lines_w<-1, EPL>(buf, nl, np2, g.ax[2], tw, lane);
 
   const double fac = 1.0 / ((double)g.np0 * (double)g.np1 * (double)g.np2);
   int n1, n2;
@@ -423,12 +441,12 @@ k_zpass_fwd(GridDev g, const cplx<T>* __restrict__ t1, const double* __restrict_
     n2 = nst;
   }
   const bool has2 = (MODE == SLOT_PAIR) && (n2 < nst);
-  const int le = threadIdx.x >> 4;
-  const int sub = threadIdx.x & 15;
+  const int le = lane >> 4;
+  const int sub = lane & 15;
   if (le >= nl) return;
   const int e = e0 + le;
   const int kind = g.ent_kind[e];
-  if (kind == 2) return;  // -G image: handled with its +G partner (next-lower line)
+  if (kind == 2 || kind == 4) return;  // -G image: handled with its +G partner (next line)
   const int rod = g.ent_rod[e];
   const int first = g.rod_first[rod];
   const int lmin = g.rod_lmin[rod];
@@ -495,13 +513,17 @@ size_t slot_bytes(const Basis* b, int prec) {
 }
 
 template <typename T>
-static size_t lds_z(const GridDev& g) { return sizeof(cplx<T>) * (size_t)g.np2 * (1 + ZL); }
-template <typename T>
-static size_t lds_y(const GridDev& g) {
-  return sizeof(cplx<T>) * (size_t)g.np1 * (1 + ZBlk<T>::ZB * ZBlk<T>::LB);
+static size_t lds_z(const GridDev& g) {
+  return sizeof(cplx<T>) * (size_t)g.np2 * (1 + NWAVE * ZLPW);
 }
 template <typename T>
-static size_t lds_x(const GridDev& g) { return sizeof(cplx<T>) * (size_t)g.np0 * (1 + YB); }
+static size_t lds_y(const GridDev& g) {
+  return sizeof(cplx<T>) * (size_t)g.np1 * (1 + Cfg<T>::ZB * Cfg<T>::LB);
+}
+template <typename T>
+static size_t lds_x(const GridDev& g) {
+  return sizeof(cplx<T>) * (size_t)g.np0 * (1 + NWAVE * Cfg<T>::LPW);
+}
 
 static int check_launch(Ctx* ctx, const char* what) {
   hipError_t e = hipGetLastError();
@@ -512,6 +534,19 @@ static int check_launch(Ctx* ctx, const char* what) {
   return 0;
 }
 
+// NMAX dispatch: 144 covers every grid edge up to 144 (9 elements per lane for 4-line
+// wavefronts), 256 the rest
+#define QBH_NMAX_DISPATCH(n, CALL) \
+  do {                             \
+    if ((n) <= 144) {              \
+      constexpr int NM = 144;      \
+      CALL;                        \
+    } else {                       \
+      constexpr int NM = 256;      \
+      CALL;                        \
+    }                              \
+  } while (0)
+
 template <typename T>
 static int bwd_yz_t(Ctx* ctx, Basis* b, const PipelineArgs& a) {
   const GridDev& g = b->g;
@@ -520,22 +555,25 @@ static int bwd_yz_t(Ctx* ctx, Basis* b, const PipelineArgs& a) {
   {
     TimedLaunch tl(ctx, "zpass_bwd");
     dim3 grid(g.ngroups, a.nslots);
+    const size_t lds = lds_z<T>(g);
     if (a.mode == SLOT_PAIR)
-      k_zpass_bwd<T, SLOT_PAIR><<<grid, NTHR, lds_z<T>(g), ctx->stream>>>(g, a.psi, a.ld, a.nst,
-                                                                           a.band0, t1);
+      QBH_NMAX_DISPATCH(g.np2, (k_zpass_bwd<T, SLOT_PAIR, NM><<<grid, NTHR, lds, ctx->stream>>>(
+                                   g, a.psi, a.ld, a.nst, a.band0, t1)));
     else if (a.mode == SLOT_SINGLE_REAL)
-      k_zpass_bwd<T, SLOT_SINGLE_REAL><<<grid, NTHR, lds_z<T>(g), ctx->stream>>>(
-          g, a.psi, a.ld, a.nst, a.band0, t1);
+      QBH_NMAX_DISPATCH(g.np2,
+                        (k_zpass_bwd<T, SLOT_SINGLE_REAL, NM><<<grid, NTHR, lds, ctx->stream>>>(
+                            g, a.psi, a.ld, a.nst, a.band0, t1)));
     else
-      k_zpass_bwd<T, SLOT_COMPLEX><<<grid, NTHR, lds_z<T>(g), ctx->stream>>>(g, a.psi, a.ld, a.nst,
-                                                                              a.band0, t1);
+      QBH_NMAX_DISPATCH(g.np2, (k_zpass_bwd<T, SLOT_COMPLEX, NM><<<grid, NTHR, lds, ctx->stream>>>(
+                                   g, a.psi, a.ld, a.nst, a.band0, t1)));
   }
   if (int r = check_launch(ctx, "zpass_bwd")) return r;
   {
     TimedLaunch tl(ctx, "ypass_bwd");
-    constexpr int ZB = ZBlk<T>::ZB, LB = ZBlk<T>::LB;
+    constexpr int ZB = Cfg<T>::ZB, LB = Cfg<T>::LB;
     dim3 grid(((g.np2 + ZB - 1) / ZB) * ((g.nlx + LB - 1) / LB), a.nslots);
-    k_ypass_bwd<T><<<grid, NTHR, lds_y<T>(g), ctx->stream>>>(g, t1, t2);
+    QBH_NMAX_DISPATCH(g.np1, (k_ypass_bwd<T, NM><<<grid, NTHR, lds_y<T>(g), ctx->stream>>>(
+                                 g, t1, t2)));
   }
   return check_launch(ctx, "ypass_bwd");
 }
@@ -552,23 +590,26 @@ static int fwd_zy_t(Ctx* ctx, Basis* b, const PipelineArgs& a, double* out, long
   auto* t2 = static_cast<cplx<T>*>(a.t2);
   {
     TimedLaunch tl(ctx, "ypass_fwd");
-    constexpr int ZB = ZBlk<T>::ZB, LB = ZBlk<T>::LB;
+    constexpr int ZB = Cfg<T>::ZB, LB = Cfg<T>::LB;
     dim3 grid(((g.np2 + ZB - 1) / ZB) * ((g.nlx + LB - 1) / LB), a.nslots);
-    k_ypass_fwd<T><<<grid, NTHR, lds_y<T>(g), ctx->stream>>>(g, t2, t1);
+    QBH_NMAX_DISPATCH(g.np1, (k_ypass_fwd<T, NM><<<grid, NTHR, lds_y<T>(g), ctx->stream>>>(
+                                 g, t2, t1)));
   }
   if (int r = check_launch(ctx, "ypass_fwd")) return r;
   {
     TimedLaunch tl(ctx, "zpass_fwd");
     dim3 grid(g.ngroups, a.nslots);
+    const size_t lds = lds_z<T>(g);
     if (a.mode == SLOT_PAIR)
-      k_zpass_fwd<T, SLOT_PAIR><<<grid, NTHR, lds_z<T>(g), ctx->stream>>>(
-          g, t1, a.psi, a.ld, a.nst, a.band0, out, ldo, kin, store);
+      QBH_NMAX_DISPATCH(g.np2, (k_zpass_fwd<T, SLOT_PAIR, NM><<<grid, NTHR, lds, ctx->stream>>>(
+                                   g, t1, a.psi, a.ld, a.nst, a.band0, out, ldo, kin, store)));
     else if (a.mode == SLOT_SINGLE_REAL)
-      k_zpass_fwd<T, SLOT_SINGLE_REAL><<<grid, NTHR, lds_z<T>(g), ctx->stream>>>(
-          g, t1, a.psi, a.ld, a.nst, a.band0, out, ldo, kin, store);
+      QBH_NMAX_DISPATCH(g.np2,
+                        (k_zpass_fwd<T, SLOT_SINGLE_REAL, NM><<<grid, NTHR, lds, ctx->stream>>>(
+                            g, t1, a.psi, a.ld, a.nst, a.band0, out, ldo, kin, store)));
     else
-      k_zpass_fwd<T, SLOT_COMPLEX><<<grid, NTHR, lds_z<T>(g), ctx->stream>>>(
-          g, t1, a.psi, a.ld, a.nst, a.band0, out, ldo, kin, store);
+      QBH_NMAX_DISPATCH(g.np2, (k_zpass_fwd<T, SLOT_COMPLEX, NM><<<grid, NTHR, lds, ctx->stream>>>(
+                                   g, t1, a.psi, a.ld, a.nst, a.band0, out, ldo, kin, store)));
   }
   return check_launch(ctx, "zpass_fwd");
 }
@@ -579,68 +620,62 @@ int pipeline_fwd_zy_accumulate(Ctx* ctx, Basis* b, const PipelineArgs& a, double
                              : fwd_zy_t<float>(ctx, b, a, out, ldo, kin, store);
 }
 
-// slots per x-pass workgroup: enough workgroups to fill the chip several times over
+// slots per x-pass workgroup: keep >= 8192 workgroups in flight while amortising the
+// potential rows and twiddles over several slots
+template <typename T>
 static int x_slots_per_wg(const GridDev& g, int nslots) {
+  constexpr int YB = Cfg<T>::LPW * NWAVE;
   const long rows = (long)g.np2 * ((g.np1 + YB - 1) / YB);
   int spw = 1;
-  while (spw < 8 && rows * ((nslots + 2 * spw - 1) / (2 * spw)) >= 4096) spw *= 2;
+  while (spw < 16 && rows * ((nslots + 2 * spw - 1) / (2 * spw)) >= 8192) spw *= 2;
   return spw;
 }
 
 template <typename T, int OP>
 static int xpass_t(Ctx* ctx, Basis* b, const PipelineArgs& a, const double* in, double* out,
-                   const double* dens, double scale, const char* name) {
+                   const double* dens, double scale, int cplx_density, const char* name) {
   const GridDev& g = b->g;
   auto* t2 = static_cast<cplx<T>*>(a.t2);
+  constexpr int YB = Cfg<T>::LPW * NWAVE;
   const int nyb = (g.np1 + YB - 1) / YB;
-  int spw = (OP == X_DENSITY) ? a.nslots : x_slots_per_wg(g, a.nslots);
+  int spw = (OP == X_DENSITY) ? a.nslots : x_slots_per_wg<T>(g, a.nslots);
   dim3 grid(g.np2 * nyb, (a.nslots + spw - 1) / spw);
   {
     TimedLaunch tl(ctx, name);
-    k_xpass<T, OP><<<grid, NTHR, lds_x<T>(g), ctx->stream>>>(g, t2, a.nslots, spw, in, out, dens,
-                                                             scale);
+    QBH_NMAX_DISPATCH(g.np0, (k_xpass<T, OP, NM><<<grid, NTHR, lds_x<T>(g), ctx->stream>>>(
+                                 g, t2, a.nslots, spw, in, out, dens, scale, cplx_density)));
   }
   return check_launch(ctx, name);
 }
 
 int pipeline_x_vmul(Ctx* ctx, Basis* b, const PipelineArgs& a, const double* v) {
   return a.prec == PREC_FP64
-             ? xpass_t<double, X_VMUL>(ctx, b, a, v, nullptr, nullptr, 1.0, "xpass_vmul")
-             : xpass_t<float, X_VMUL>(ctx, b, a, v, nullptr, nullptr, 1.0, "xpass_vmul");
+             ? xpass_t<double, X_VMUL>(ctx, b, a, v, nullptr, nullptr, 1.0, 0, "xpass_vmul")
+             : xpass_t<float, X_VMUL>(ctx, b, a, v, nullptr, nullptr, 1.0, 0, "xpass_vmul");
 }
 int pipeline_x_store(Ctx* ctx, Basis* b, const PipelineArgs& a, double* f) {
   return a.prec == PREC_FP64
-             ? xpass_t<double, X_STORE>(ctx, b, a, nullptr, f, nullptr, 1.0, "xpass_store")
-             : xpass_t<float, X_STORE>(ctx, b, a, nullptr, f, nullptr, 1.0, "xpass_store");
+             ? xpass_t<double, X_STORE>(ctx, b, a, nullptr, f, nullptr, 1.0, 0, "xpass_store")
+             : xpass_t<float, X_STORE>(ctx, b, a, nullptr, f, nullptr, 1.0, 0, "xpass_store");
 }
 int pipeline_x_load(Ctx* ctx, Basis* b, const PipelineArgs& a, const double* f, int real_input,
                     double scale) {
   if (real_input)
     return a.prec == PREC_FP64
-               ? xpass_t<double, X_LOAD_REAL>(ctx, b, a, f, nullptr, nullptr, scale, "xpass_load")
-               : xpass_t<float, X_LOAD_REAL>(ctx, b, a, f, nullptr, nullptr, scale, "xpass_load");
+               ? xpass_t<double, X_LOAD_REAL>(ctx, b, a, f, nullptr, nullptr, scale, 0,
+                                              "xpass_load")
+               : xpass_t<float, X_LOAD_REAL>(ctx, b, a, f, nullptr, nullptr, scale, 0,
+                                             "xpass_load");
   return a.prec == PREC_FP64
-             ? xpass_t<double, X_LOAD>(ctx, b, a, f, nullptr, nullptr, 1.0, "xpass_load")
-             : xpass_t<float, X_LOAD>(ctx, b, a, f, nullptr, nullptr, 1.0, "xpass_load");
+             ? xpass_t<double, X_LOAD>(ctx, b, a, f, nullptr, nullptr, 1.0, 0, "xpass_load")
+             : xpass_t<float, X_LOAD>(ctx, b, a, f, nullptr, nullptr, 1.0, 0, "xpass_load");
 }
 int pipeline_x_density(Ctx* ctx, Basis* b, const PipelineArgs& a, const double* dens,
                        double* rho) {
-  const GridDev& g = b->g;
-  if (a.mode == SLOT_COMPLEX) {
-    const int nyb = (g.np1 + YB - 1) / YB;
-    dim3 grid(g.np2 * nyb, 1);
-    TimedLaunch tl(ctx, "xpass_density");
-    if (a.prec == PREC_FP64)
-      k_xpass_density_complex<double><<<grid, NTHR, lds_x<double>(g), ctx->stream>>>(
-          g, static_cast<cplx<double>*>(a.t2), a.nslots, rho, dens);
-    else
-      k_xpass_density_complex<float><<<grid, NTHR, lds_x<float>(g), ctx->stream>>>(
-          g, static_cast<cplx<float>*>(a.t2), a.nslots, rho, dens);
-    return check_launch(ctx, "xpass_density");
-  }
+  const int cd = a.mode == SLOT_COMPLEX ? 1 : 0;
   return a.prec == PREC_FP64
-             ? xpass_t<double, X_DENSITY>(ctx, b, a, nullptr, rho, dens, 1.0, "xpass_density")
-             : xpass_t<float, X_DENSITY>(ctx, b, a, nullptr, rho, dens, 1.0, "xpass_density");
+             ? xpass_t<double, X_DENSITY>(ctx, b, a, nullptr, rho, dens, 1.0, cd, "xpass_density")
+             : xpass_t<float, X_DENSITY>(ctx, b, a, nullptr, rho, dens, 1.0, cd, "xpass_density");
 }
 
 }  // namespace qbh

@@ -12,6 +12,7 @@
 #include <hip/hip_runtime.h>
 #include <rocblas/rocblas.h>
 
+#include <algorithm>
 #include <cstdio>
 #include <string>
 
@@ -82,9 +83,10 @@ k_anl(int ngw, const int* __restrict__ col_l, const int* __restrict__ col_tw,
 // Enl partial sums, F *= wt/Omega.  anl / psi are the real views (stride 2ngw / bstride).
 template <typename T>
 __global__ void __launch_bounds__(256)
-k_fnl_finish_real(int P, int nst, T* __restrict__ F, const T* __restrict__ anl, long astride,
-                  const T* __restrict__ psi, long bstride, const double* __restrict__ fac,
-                  const double* __restrict__ occ, double* __restrict__ partial) {
+k_fnl_finish_real(int P, int nst, T* __restrict__ F, int nsplit, long pstride,
+                  const T* __restrict__ anl, long astride, const T* __restrict__ psi, long bstride,
+                  const double* __restrict__ fac, const double* __restrict__ occ,
+                  double* __restrict__ partial) {
   __shared__ double red[256];
   const long total = (long)P * nst;
   double e = 0.0;
@@ -93,7 +95,9 @@ k_fnl_finish_real(int P, int nst, T* __restrict__ F, const T* __restrict__ anl, 
     const int n = (int)(i / P);
     const T a0 = anl[(long)p * astride];
     const T c0 = psi[(long)n * bstride];
-    T f = F[i] + a0 * (T(-0.5) * c0);
+    T f = F[i];
+    for (int k = 1; k < nsplit; ++k) f += F[k * pstride + i];  // split-K partial sums
+    f = f + a0 * (T(-0.5) * c0);
     f = T(2) * f;
     const double fp = fac[p];
     const double fd = double(f);
@@ -112,8 +116,9 @@ k_fnl_finish_real(int P, int nst, T* __restrict__ F, const T* __restrict__ anl, 
 // complex finish: Enl partial sums of occ*wt/Omega*|F|^2, F *= wt/Omega.
 template <typename T>
 __global__ void __launch_bounds__(256)
-k_fnl_finish_cplx(int P, int nst, T* __restrict__ F, const double* __restrict__ fac,
-                  const double* __restrict__ occ, double* __restrict__ partial) {
+k_fnl_finish_cplx(int P, int nst, T* __restrict__ F, int nsplit, long pstride,
+                  const double* __restrict__ fac, const double* __restrict__ occ,
+                  double* __restrict__ partial) {
   __shared__ double red[256];
   const long total = (long)P * nst;
   double e = 0.0;
@@ -121,10 +126,15 @@ k_fnl_finish_cplx(int P, int nst, T* __restrict__ F, const double* __restrict__ 
     const int p = (int)(i % P);
     const int n = (int)(i / P);
     const double fp = fac[p];
-    const double fr = double(F[2 * i]), fi = double(F[2 * i + 1]);
+    T xr = F[2 * i], xi = F[2 * i + 1];
+    for (int k = 1; k < nsplit; ++k) {  // split-K partial sums
+      xr += F[k * pstride + 2 * i];
+      xi += F[k * pstride + 2 * i + 1];
+    }
+    const double fr = double(xr), fi = double(xi);
     e += (fp * occ[n]) * (fr * fr + fi * fi);
-    F[2 * i] = T(fp) * F[2 * i];
-    F[2 * i + 1] = T(fp) * F[2 * i + 1];
+    F[2 * i] = T(fp) * xr;
+    F[2 * i + 1] = T(fp) * xi;
   }
   red[threadIdx.x] = e;
   __syncthreads();
@@ -215,6 +225,74 @@ static int ensure_anl(Ham* h, bool fp32) {
   return 0;
 }
 
+// rocBLAS dispatch by element type (real: T, complex: interleaved T pairs)
+static rocblas_status gemm_sb(rocblas_handle h, rocblas_operation ta, rocblas_operation tb, int m,
+                              int n, int k, const double* A, long lda, long sa, const double* B,
+                              long ldb, long sb, double* C, long ldc, long sc, int batch,
+                              bool cplx, double beta) {
+  if (!cplx) {
+    const double one = 1.0;
+    return rocblas_dgemm_strided_batched(h, ta, tb, m, n, k, &one, A, lda, sa, B, ldb, sb, &beta, C,
+                                         ldc, sc, batch);
+  }
+  const rocblas_double_complex one(1.0, 0.0), b(beta, 0.0);
+  return rocblas_zgemm_strided_batched(
+      h, ta, tb, m, n, k, &one, reinterpret_cast<const rocblas_double_complex*>(A), lda, sa,
+      reinterpret_cast<const rocblas_double_complex*>(B), ldb, sb, &b,
+      reinterpret_cast<rocblas_double_complex*>(C), ldc, sc, batch);
+}
+static rocblas_status gemm_sb(rocblas_handle h, rocblas_operation ta, rocblas_operation tb, int m,
+                              int n, int k, const float* A, long lda, long sa, const float* B,
+                              long ldb, long sb, float* C, long ldc, long sc, int batch, bool cplx,
+                              double beta) {
+  if (!cplx) {
+    const float one = 1.f, bt = float(beta);
+    return rocblas_sgemm_strided_batched(h, ta, tb, m, n, k, &one, A, lda, sa, B, ldb, sb, &bt, C,
+                                         ldc, sc, batch);
+  }
+  const rocblas_float_complex one(1.f, 0.f), b(float(beta), 0.f);
+  return rocblas_cgemm_strided_batched(
+      h, ta, tb, m, n, k, &one, reinterpret_cast<const rocblas_float_complex*>(A), lda, sa,
+      reinterpret_cast<const rocblas_float_complex*>(B), ldb, sb, &b,
+      reinterpret_cast<rocblas_float_complex*>(C), ldc, sc, batch);
+}
+
+// Number of K-splits for F = anl^H psi: the P x nst output alone gives too few 128x128
+// tiles to fill 256 CUs when P, nst ~ 1e3 and K ~ 1e5, so K is cut into slices whose
+// partial products are summed in the finish kernel.
+static int fnl_splits(int P, int nst, long K) {
+  const long tiles = (long)((P + 127) / 128) * ((nst + 127) / 128);
+  long s = (1024 + tiles - 1) / tiles;
+  s = std::min<long>(s, 16);
+  s = std::min<long>(s, std::max<long>(1, K / 1024));
+  return (int)std::max<long>(1, s);
+}
+
+// F (nsplit partials, stride P*nst elements) = anl^T psi (real view) or anl^H psi.
+// Kelem = contraction length in elements (2*ngw reals or ngw complex).
+template <typename T>
+static int fnl_gemm(Ctx* c, rocblas_handle bh, bool cplx, int P, int nst, long Kelem, const T* anl,
+                    long lda, const T* psi, long ldb, T* F, int nsplit) {
+  const rocblas_operation ta =
+      cplx ? rocblas_operation_conjugate_transpose : rocblas_operation_transpose;
+  const int e = cplx ? 2 : 1;  // T per element
+  const long kc = (Kelem + nsplit - 1) / nsplit;
+  const long nfull = Kelem / kc;
+  const long rem = Kelem - nfull * kc;
+  const long pstride = (long)P * nst * e;
+  rocblas_status st = rocblas_status_success;
+  // batch strides are in BLAS elements (complex elements for zgemm/cgemm); pointer
+  // offsets are in T
+  if (nfull > 0)
+    st = gemm_sb(bh, ta, rocblas_operation_none, P, nst, (int)kc, anl, lda, kc, psi, ldb, kc, F,
+                 P, (long)P * nst, (int)nfull, cplx, 0.0);
+  if (st == rocblas_status_success && rem > 0)
+    st = gemm_sb(bh, ta, rocblas_operation_none, P, nst, (int)rem, anl + nfull * kc * e, lda, 0,
+                 psi + nfull * kc * e, ldb, 0, F + nfull * pstride, P, (long)P * nst, 1, cplx,
+                 0.0);
+  return blas_ok(c, st, "fnl gemm");
+}
+
 int nl_apply(Ham* h, const double* psi, long ld, int nst, const double* occ_host, double* hpsi,
              long ldh, int fp32, double* enl) {
   Ctx* c = h->ctx;
@@ -224,6 +302,7 @@ int nl_apply(Ham* h, const double* psi, long ld, int nst, const double* occ_host
   }
   const int ngw = h->b->g.ngw;
   const bool real = h->b->g.real;
+  const bool cplx = !real;
   const int P = h->P;
   if (int r = ensure_anl(h, fp32)) return r;
   void* hb;
@@ -256,61 +335,38 @@ int nl_apply(Ham* h, const double* psi, long ld, int nst, const double* occ_host
     }
   }
 
-  const size_t fsz = (size_t)P * nst * (real ? 1 : 2);
+  const long Kelem = real ? 2L * ngw : (long)ngw;  // contraction length in elements
+  const int e = cplx ? 2 : 1;
+  const int nsplit = fnl_splits(P, nst, Kelem);
+  const long pstride = (long)P * nst * e;
+  const size_t fsz = (size_t)pstride * nsplit;
   if (!fp32) {
     double* F = static_cast<double*>(grow(h->f64, h->f64_bytes, fsz * sizeof(double)));
     const double* anl = static_cast<const double*>(h->anl64);
     if (!F) return QBH_ERR_ALLOC;
-    if (real) {
-      const double one = 1.0, zero = 0.0;
-      {
-        TimedLaunch tl(c, "fnl_gemm");
-        if (int r = blas_ok(c,
-                            rocblas_dgemm(bh, rocblas_operation_transpose, rocblas_operation_none,
-                                          P, nst, 2 * ngw, &one, anl, 2 * ngw, psi, 2 * ld,
-                                          &zero, F, P),
-                            "fnl dgemm"))
-          return r;
-      }
-      {
-        TimedLaunch tl(c, "fnl_finish");
-        k_fnl_finish_real<double><<<nblk, 256, 0, c->stream>>>(P, nst, F, anl, 2L * ngw, psi,
-                                                               2L * ld, h->col_fac, occ, h->red);
-      }
-      TimedLaunch tl(c, "enl_gemm");
-      if (int r = blas_ok(c,
-                          rocblas_dgemm(bh, rocblas_operation_none, rocblas_operation_none,
-                                        2 * ngw, nst, P, &one, anl, 2 * ngw, F, P, &one, hpsi,
-                                        2 * ldh),
-                          "enl dgemm"))
-        return r;
-    } else {
-      const rocblas_double_complex one(1.0, 0.0), zero(0.0, 0.0);
-      auto* A = reinterpret_cast<const rocblas_double_complex*>(anl);
-      auto* B = reinterpret_cast<const rocblas_double_complex*>(psi);
-      auto* Fz = reinterpret_cast<rocblas_double_complex*>(F);
-      {
-        TimedLaunch tl(c, "fnl_gemm");
-        if (int r = blas_ok(c,
-                            rocblas_zgemm(bh, rocblas_operation_conjugate_transpose,
-                                          rocblas_operation_none, P, nst, ngw, &one, A, ngw, B,
-                                          ld, &zero, Fz, P),
-                            "fnl zgemm"))
-          return r;
-      }
-      {
-        TimedLaunch tl(c, "fnl_finish");
-        k_fnl_finish_cplx<double><<<nblk, 256, 0, c->stream>>>(P, nst, F, h->col_fac, occ,
-                                                               h->red);
-      }
-      TimedLaunch tl(c, "enl_gemm");
-      if (int r = blas_ok(c,
-                          rocblas_zgemm(bh, rocblas_operation_none, rocblas_operation_none, ngw,
-                                        nst, P, &one, A, ngw, Fz, P, &one,
-                                        reinterpret_cast<rocblas_double_complex*>(hpsi), ldh),
-                          "enl zgemm"))
+    {
+      TimedLaunch tl(c, "fnl_gemm");
+      if (int r = fnl_gemm<double>(c, bh, cplx, P, nst, Kelem, anl, Kelem, psi,
+                                   real ? 2 * ld : ld, F, nsplit))
         return r;
     }
+    {
+      TimedLaunch tl(c, "fnl_finish");
+      if (real)
+        k_fnl_finish_real<double><<<nblk, 256, 0, c->stream>>>(P, nst, F, nsplit, pstride, anl,
+                                                               2L * ngw, psi, 2L * ld, h->col_fac,
+                                                               occ, h->red);
+      else
+        k_fnl_finish_cplx<double><<<nblk, 256, 0, c->stream>>>(P, nst, F, nsplit, pstride,
+                                                               h->col_fac, occ, h->red);
+    }
+    TimedLaunch tl(c, "enl_gemm");
+    if (int r = blas_ok(c,
+                        gemm_sb(bh, rocblas_operation_none, rocblas_operation_none,
+                                (int)Kelem, nst, P, anl, Kelem, 0, F, P, 0, hpsi,
+                                real ? 2 * ldh : ldh, 0, 1, cplx, 1.0),
+                        "enl gemm"))
+      return r;
   } else {
     float* F = static_cast<float*>(grow(h->f32, h->f32_bytes, fsz * sizeof(float)));
     const size_t psz = 2 * (size_t)ngw * nst;
@@ -325,54 +381,27 @@ int nl_apply(Ham* h, const double* psi, long ld, int nst, const double* occ_host
       TimedLaunch tl(c, "to_f32");
       k_to_f32<<<(unsigned)((psz + 255) / 256), 256, 0, c->stream>>>(ngw, nst, psi, ld, p32);
     }
-    if (real) {
-      const float one = 1.f, zero = 0.f;
-      {
-        TimedLaunch tl(c, "fnl_gemm");
-        if (int r = blas_ok(c,
-                            rocblas_sgemm(bh, rocblas_operation_transpose, rocblas_operation_none,
-                                          P, nst, 2 * ngw, &one, anl, 2 * ngw, p32, 2 * ngw,
-                                          &zero, F, P),
-                            "fnl sgemm"))
-          return r;
-      }
-      {
-        TimedLaunch tl(c, "fnl_finish");
-        k_fnl_finish_real<float><<<nblk, 256, 0, c->stream>>>(P, nst, F, anl, 2L * ngw, p32,
-                                                              2L * ngw, h->col_fac, occ, h->red);
-      }
-      TimedLaunch tl(c, "enl_gemm");
-      if (int r = blas_ok(c,
-                          rocblas_sgemm(bh, rocblas_operation_none, rocblas_operation_none,
-                                        2 * ngw, nst, P, &one, anl, 2 * ngw, F, P, &zero, t32,
-                                        2 * ngw),
-                          "enl sgemm"))
+    {
+      TimedLaunch tl(c, "fnl_gemm");
+      if (int r = fnl_gemm<float>(c, bh, cplx, P, nst, Kelem, anl, Kelem, p32, Kelem, F, nsplit))
         return r;
-    } else {
-      const rocblas_float_complex one(1.f, 0.f), zero(0.f, 0.f);
-      auto* A = reinterpret_cast<const rocblas_float_complex*>(anl);
-      auto* B = reinterpret_cast<const rocblas_float_complex*>(p32);
-      auto* Fz = reinterpret_cast<rocblas_float_complex*>(F);
-      {
-        TimedLaunch tl(c, "fnl_gemm");
-        if (int r = blas_ok(c,
-                            rocblas_cgemm(bh, rocblas_operation_conjugate_transpose,
-                                          rocblas_operation_none, P, nst, ngw, &one, A, ngw, B,
-                                          ngw, &zero, Fz, P),
-                            "fnl cgemm"))
-          return r;
-      }
-      {
-        TimedLaunch tl(c, "fnl_finish");
-        k_fnl_finish_cplx<float><<<nblk, 256, 0, c->stream>>>(P, nst, F, h->col_fac, occ,
-                                                              h->red);
-      }
+    }
+    {
+      TimedLaunch tl(c, "fnl_finish");
+      if (real)
+        k_fnl_finish_real<float><<<nblk, 256, 0, c->stream>>>(P, nst, F, nsplit, pstride, anl,
+                                                              2L * ngw, p32, 2L * ngw, h->col_fac,
+                                                              occ, h->red);
+      else
+        k_fnl_finish_cplx<float><<<nblk, 256, 0, c->stream>>>(P, nst, F, nsplit, pstride,
+                                                              h->col_fac, occ, h->red);
+    }
+    {
       TimedLaunch tl(c, "enl_gemm");
       if (int r = blas_ok(c,
-                          rocblas_cgemm(bh, rocblas_operation_none, rocblas_operation_none, ngw,
-                                        nst, P, &one, A, ngw, Fz, P, &zero,
-                                        reinterpret_cast<rocblas_float_complex*>(t32), ngw),
-                          "enl cgemm"))
+                          gemm_sb(bh, rocblas_operation_none, rocblas_operation_none, (int)Kelem,
+                                  nst, P, anl, Kelem, 0, F, P, 0, t32, Kelem, 0, 1, cplx, 0.0),
+                          "enl gemm"))
         return r;
     }
     {
@@ -380,9 +409,9 @@ int nl_apply(Ham* h, const double* psi, long ld, int nst, const double* occ_host
       k_add_f32<<<(unsigned)((psz + 255) / 256), 256, 0, c->stream>>>(ngw, nst, t32, hpsi, ldh);
     }
   }
-  hipError_t e = hipGetLastError();
-  if (e != hipSuccess) {
-    c->last_error = std::string("nonlocal: ") + hipGetErrorString(e);
+  hipError_t err = hipGetLastError();
+  if (err != hipSuccess) {
+    c->last_error = std::string("nonlocal: ") + hipGetErrorString(err);
     return QBH_ERR_DEVICE;
   }
   if (enl) {
