@@ -1,0 +1,119 @@
+"""Oracle transforms pinned against brute-force direct sums and closed forms."""
+import numpy as np
+import pytest
+
+from oracle import fft as offt
+from oracle.basis import Basis, FFTMaps, UnitCell, factorizable
+
+
+def small(real=True):
+    cell = UnitCell(np.array([[6.0, 0.2, 0.0], [0.0, 5.5, 0.4], [0.3, 0.0, 6.5]]))
+    b = Basis(cell, 5.0, force_complex=not real)
+    n = [x for x in b.np]
+    return b, FFTMaps(b, *n)
+
+
+def direct_backward(b, maps, zfull):
+    """f(x,y,z) = sum over stored (G, value) pairs of value * exp(+2 pi i (hx/n0+ky/n1+lz/n2))."""
+    n0, n1, n2 = maps.np0, maps.np1, maps.np2
+    z, y, x = np.meshgrid(np.arange(n2), np.arange(n1), np.arange(n0), indexing="ij")
+    f = np.zeros((n2, n1, n0), dtype=np.complex128)
+    for (h, k, l), c in zfull:
+        f += c * np.exp(2j * np.pi * (h * x / n0 + k * y / n1 + l * z / n2))
+    return f
+
+
+def sphere_terms(b, c):
+    """(G, value) list of the Γ expansion: c(G) at G and conj(c(G)) at -G (G != 0)."""
+    out = []
+    for i, (h, k, l) in enumerate(b.idx):
+        out.append(((h, k, l), c[i]))
+        if b.real and not (h == 0 and k == 0 and l == 0):
+            out.append(((-h, -k, -l), np.conj(c[i])))
+    return out
+
+
+def test_backward_single_direct_sum():
+    b, maps = small(True)
+    rng = np.random.default_rng(1)
+    c = rng.standard_normal(b.size) + 1j * rng.standard_normal(b.size)
+    c[0] = c[0].real
+    f = offt.backward(maps, c)
+    ref = direct_backward(b, maps, sphere_terms(b, c))
+    assert np.max(np.abs(f - ref)) <= 1e-12 * np.max(np.abs(ref))
+    assert np.max(np.abs(f.imag)) <= 1e-12 * np.max(np.abs(f))   # Γ: real function
+
+
+def test_backward_pair_is_psi1_plus_i_psi2():
+    b, maps = small(True)
+    rng = np.random.default_rng(2)
+    c1 = rng.standard_normal(b.size) + 1j * rng.standard_normal(b.size)
+    c2 = rng.standard_normal(b.size) + 1j * rng.standard_normal(b.size)
+    c1[0], c2[0] = c1[0].real, c2[0].real
+    f = offt.backward(maps, c1, c2)
+    r1 = direct_backward(b, maps, sphere_terms(b, c1))
+    r2 = direct_backward(b, maps, sphere_terms(b, c2))
+    assert np.max(np.abs(f - (r1 + 1j * r2))) <= 1e-12 * np.max(np.abs(r1))
+
+
+def test_backward_complex_basis_direct_sum():
+    b, maps = small(False)
+    rng = np.random.default_rng(3)
+    c = rng.standard_normal(b.size) + 1j * rng.standard_normal(b.size)
+    f = offt.backward(maps, c)
+    ref = direct_backward(b, maps, [(tuple(g), c[i]) for i, g in enumerate(b.idx)])
+    assert np.max(np.abs(f - ref)) <= 1e-12 * np.max(np.abs(ref))
+
+
+def test_gamma_g0_conjugate_wins():
+    """backward(c) puts conj(c(0)) at G=0 (the -G write is second)."""
+    b, maps = small(True)
+    c = np.zeros(b.size, dtype=np.complex128)
+    c[0] = 1.0 + 2.0j
+    f = offt.backward(maps, c)
+    assert np.allclose(f, 1.0 - 2.0j, atol=1e-14)
+
+
+def test_plane_wave_closed_form():
+    b, maps = small(True)
+    i = 7
+    c = np.zeros(b.size, dtype=np.complex128)
+    c[i] = 0.5 - 0.25j
+    f = offt.backward(maps, c)
+    h, k, l = b.idx[i]
+    n0, n1, n2 = maps.np0, maps.np1, maps.np2
+    z, y, x = np.meshgrid(np.arange(n2), np.arange(n1), np.arange(n0), indexing="ij")
+    ph = 2 * np.pi * (h * x / n0 + k * y / n1 + l * z / n2)
+    ref = 2 * (0.5 * np.cos(ph) + 0.25 * np.sin(ph))
+    assert np.max(np.abs(f - ref)) <= 1e-13
+
+
+def test_forward_direct_projection_and_scale():
+    b, maps = small(True)
+    rng = np.random.default_rng(4)
+    n0, n1, n2 = maps.np0, maps.np1, maps.np2
+    f = rng.standard_normal((n2, n1, n0)) + 1j * rng.standard_normal((n2, n1, n0))
+    c = offt.forward(maps, f)
+    z, y, x = np.meshgrid(np.arange(n2), np.arange(n1), np.arange(n0), indexing="ij")
+    for i in (0, 3, 11, b.size - 1):
+        h, k, l = b.idx[i]
+        ref = np.sum(f * np.exp(-2j * np.pi * (h * x / n0 + k * y / n1 + l * z / n2))) / f.size
+        assert abs(c[i] - ref) <= 1e-13
+
+
+def test_pair_round_trip():
+    b, maps = small(True)
+    rng = np.random.default_rng(5)
+    c1 = rng.standard_normal(b.size) + 1j * rng.standard_normal(b.size)
+    c2 = rng.standard_normal(b.size) + 1j * rng.standard_normal(b.size)
+    c1[0], c2[0] = c1[0].real, c2[0].real
+    d1, d2 = offt.forward(maps, offt.backward(maps, c1, c2), pair=True)
+    assert np.max(np.abs(d1 - c1)) <= 1e-14 and np.max(np.abs(d2 - c2)) <= 1e-14
+
+
+def test_parseval():
+    b, maps = small(False)
+    rng = np.random.default_rng(6)
+    c = rng.standard_normal(b.size) + 1j * rng.standard_normal(b.size)
+    f = offt.backward(maps, c)
+    assert abs(np.sum(np.abs(f) ** 2) / f.size - np.sum(np.abs(c) ** 2)) <= 1e-12 * np.sum(np.abs(c) ** 2)
