@@ -149,16 +149,22 @@ def main():
         for _ in range(args.warmup):
             step(QBH_ALL | QBH_FP32)
         torch.cuda.synchronize()
+        ctx.timing_report()
+        ctx.set_timing(True)
         t0 = time.perf_counter()
         for _ in range(args.steps):
             step(QBH_ALL | QBH_FP32)
         torch.cuda.synchronize()
         dt32 = (time.perf_counter() - t0) / args.steps
+        ctx.set_timing(False)
+        k32 = {k: {"avg_ms": ms / cnt, "launches_per_step": cnt / args.steps}
+               for k, (ms, cnt) in ctx.timing_report().items()}
         e32 = band_energy(hpsi)
         rel = float(torch.linalg.vector_norm(hpsi - h64) / torch.linalg.vector_norm(h64))
         fp32 = {"value": nst / dt32, "ms_per_step": dt32 * 1e3, "speedup_vs_fp64": dt / dt32,
                 "hpsi_rel_err_vs_fp64": rel,
-                "band_energy_diff_per_atom_Ha": abs(e32 - e64) / sysm.na}
+                "band_energy_diff_per_atom_Ha": abs(e32 - e64) / sysm.na,
+                "kernels": k32}
         del h64
 
     # CPU baseline: the oracle (numpy restatement) on a bounded sample, 1 thread

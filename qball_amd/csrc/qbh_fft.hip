@@ -18,6 +18,7 @@
 #include <algorithm>
 #include <cstdio>
 
+#include "fft_reg.h"
 #include "qbh_internal.h"
 
 namespace qbh {
@@ -401,6 +402,189 @@ k_xpass(GridDev g, cplx<T>* __restrict__ t2, int nslots, int slots_per_wg,
 }
 
 // ------------------------------------------------------------------------------------
+// x pass with the two-factor register transform (fft_reg.h), rows of n = N1*N2:
+//   stage A  (bwd): DFT_N1 over stride-N2 elements, twiddle W_n^{j2 k1}
+//   stage B  (bwd): DFT_N2 over contiguous blocks -> psi(r) at x = k1 + N1*k2, in
+//                   registers; VMUL multiplies by v(r) there and runs stage B of the
+//                   forward transform (DFT_N2, twiddle W_n^{-k1 m2}) without leaving
+//                   registers; DENSITY accumulates |psi|^2 there
+//   stage A' (fwd): DFT_N1 over stride-N2 elements -> natural order
+// Three LDS round trips per slot instead of one per radix pass, and the x-dependent
+// multiply touches each element once, in registers.
+template <typename T, int OP, int N1, int N2>
+__global__ void __launch_bounds__(NTHR)
+k_xpass2(GridDev g, cplx<T>* __restrict__ t2, int nslots, int slots_per_wg,
+         const double* __restrict__ v, double* __restrict__ out, const double* __restrict__ dens,
+         int cplx_density) {
+  constexpr int LPW = Cfg<T>::LPW;
+  constexpr int YB = LPW * NWAVE;
+  constexpr int n = N1 * N2;
+  constexpr int TA = LPW * N2, TB = LPW * N1;
+  constexpr int IA = (TA + 63) / 64, IB = (TB + 63) / 64;
+  constexpr int GPL = (LPW * n + 63) / 64;
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  const int np1 = g.np1, np2 = g.np2, nlx = g.nlx;
+  cplx<T>* tw = reinterpret_cast<cplx<T>*>(smem_raw);
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int nyb = (np1 + YB - 1) / YB;
+  const int z = blockIdx.x / nyb;
+  const int y0 = (blockIdx.x % nyb) * YB + wave * LPW;
+  const int nyl = min(LPW, np1 - y0);
+  const int s0 = blockIdx.y * slots_per_wg;
+  const int s1 = min(nslots, s0 + slots_per_wg);
+  const cplx<T>* twg = twtab<T>(g, 0);
+  for (int i = threadIdx.x; i < n; i += NTHR) tw[i] = twg[i];
+  __syncthreads();
+  if (nyl <= 0) return;
+  cplx<T>* buf = tw + n + wave * LPW * n;
+  const long grow0 = ((long)z * np1 + y0) * n;
+  const int ng = nlx * LPW;
+
+  T vr[IB][N2];
+  double acc[IB][N2];
+#pragma unroll
+  for (int ib = 0; ib < IB; ++ib) {
+    const int t = lane + 64 * ib;
+    const int r = t / N1, k1 = t % N1;
+#pragma unroll
+    for (int k2 = 0; k2 < N2; ++k2) {
+      if (OP == X_VMUL) vr[ib][k2] = (t < TB && r < nyl) ? T(v[grow0 + r * n + k1 + N1 * k2]) : T(0);
+      if (OP == X_DENSITY) acc[ib][k2] = 0.0;
+    }
+  }
+
+  cplx<T> pf[GPL];
+  auto prefetch = [&](int s) {
+#pragma unroll
+    for (int k = 0; k < GPL; ++k) {
+      const int q = lane + k * 64;
+      const int dy = q % LPW, lx = q / LPW;
+      pf[k] = (q < ng && dy < nyl) ? t2[(((long)s * np2 + z) * nlx + lx) * np1 + y0 + dy]
+                                   : cplx<T>{T(0), T(0)};
+    }
+  };
+  if (s0 < s1) prefetch(s0);
+
+  for (int s = s0; s < s1; ++s) {
+    for (int i = lane; i < LPW * n; i += 64) buf[i] = {T(0), T(0)};
+    wave_sync();
+#pragma unroll
+    for (int k = 0; k < GPL; ++k) {
+      const int q = lane + k * 64;
+      if (q < ng) buf[(q % LPW) * n + g.line_x[q / LPW]] = pf[k];
+    }
+    wave_sync();
+    if (s + 1 < s1) prefetch(s + 1);
+
+    // stage A, backward
+#pragma unroll
+    for (int ia = 0; ia < IA; ++ia) {
+      const int t = lane + 64 * ia;
+      if (t < TA) {
+        const int r = t / N2, j2 = t % N2;
+        cplx<T>* row = buf + r * n + j2;
+        cplx<T> a[N1];
+#pragma unroll
+        for (int j1 = 0; j1 < N1; ++j1) a[j1] = row[N2 * j1];
+        sdft<N1, +1>(a);
+        twiddle_row<N1, +1>(a, tw, j2);
+#pragma unroll
+        for (int k1 = 0; k1 < N1; ++k1) row[N2 * k1] = a[k1];
+      }
+    }
+    wave_sync();
+    // stage B: backward DFT_N2 -> psi(r); multiply / accumulate; forward DFT_N2
+    double f1 = 0, f2 = 0, b1 = 0, b2 = 0;
+    if (OP == X_DENSITY) {
+      if (cplx_density) {
+        f1 = dens[s];
+      } else {
+        f1 = dens[4 * s + 0];
+        f2 = dens[4 * s + 1];
+        b1 = dens[4 * s + 2];
+        b2 = dens[4 * s + 3];
+      }
+    }
+#pragma unroll
+    for (int ib = 0; ib < IB; ++ib) {
+      const int t = lane + 64 * ib;
+      if (t < TB) {
+        const int r = t / N1, k1 = t % N1;
+        cplx<T>* blk = buf + r * n + N2 * k1;
+        cplx<T> b[N2];
+#pragma unroll
+        for (int j2 = 0; j2 < N2; ++j2) b[j2] = blk[j2];
+        sdft<N2, +1>(b);
+        if (OP == X_VMUL) {
+#pragma unroll
+          for (int k2 = 0; k2 < N2; ++k2) b[k2] = {vr[ib][k2] * b[k2].x, vr[ib][k2] * b[k2].y};
+          sdft<N2, -1>(b);
+          twiddle_row<N2, -1>(b, tw, k1);
+#pragma unroll
+          for (int m2 = 0; m2 < N2; ++m2) blk[m2] = b[m2];
+        } else {
+#pragma unroll
+          for (int k2 = 0; k2 < N2; ++k2) {
+            const double re = double(b[k2].x), im = double(b[k2].y);
+            if (cplx_density) {
+              if (f1 > 0.0) acc[ib][k2] += f1 * (re * re + im * im);
+            } else {
+              // f = psi1 + i psi2, psi_j = X_j + conj(c_j(G=0)) (see k_xpass)
+              if (f1 > 0.0) {
+                const double r1 = re - b2;
+                acc[ib][k2] += f1 * (r1 * r1 + b1 * b1);
+              }
+              if (f2 > 0.0) {
+                const double r2 = im + b1;
+                acc[ib][k2] += f2 * (r2 * r2 + b2 * b2);
+              }
+            }
+          }
+        }
+      }
+    }
+    if (OP == X_DENSITY) {
+      wave_sync();
+      continue;
+    }
+    wave_sync();
+    // stage A', forward: natural order
+#pragma unroll
+    for (int ia = 0; ia < IA; ++ia) {
+      const int t = lane + 64 * ia;
+      if (t < TA) {
+        const int r = t / N2, m2 = t % N2;
+        cplx<T>* row = buf + r * n + m2;
+        cplx<T> a[N1];
+#pragma unroll
+        for (int k1 = 0; k1 < N1; ++k1) a[k1] = row[N2 * k1];
+        sdft<N1, -1>(a);
+#pragma unroll
+        for (int m1 = 0; m1 < N1; ++m1) row[N2 * m1] = a[m1];
+      }
+    }
+    wave_sync();
+    for (int q = lane; q < ng; q += 64) {
+      const int dy = q % LPW, lx = q / LPW;
+      if (dy < nyl)
+        t2[(((long)s * np2 + z) * nlx + lx) * np1 + y0 + dy] = buf[dy * n + g.line_x[lx]];
+    }
+    wave_sync();
+  }
+  if (OP == X_DENSITY) {
+#pragma unroll
+    for (int ib = 0; ib < IB; ++ib) {
+      const int t = lane + 64 * ib;
+      const int r = t / N1, k1 = t % N1;
+      if (t < TB && r < nyl) {
+#pragma unroll
+        for (int k2 = 0; k2 < N2; ++k2) out[grow0 + r * n + k1 + N1 * k2] += acc[ib][k2];
+      }
+    }
+  }
+}
+
+// ------------------------------------------------------------------------------------
 // z pass, forward: T1 columns -> z transform (exp(-i)) -> x 1/N -> unpack -> output.
 // out[n][G] = value (store) or out[n][G] = (out + kin[G]*psi[n][G]) + value.
 template <typename T, int MODE, int NMAX>
@@ -631,6 +815,31 @@ static int x_slots_per_wg(const GridDev& g, int nslots) {
   return spw;
 }
 
+// two-factor register x pass for the grid edges it is instantiated for
+template <typename T, int OP>
+static bool launch_x2(Ctx* ctx, const GridDev& g, dim3 grid, size_t lds, cplx<T>* t2, int nslots,
+                      int spw, const double* in, double* out, const double* dens, int cd) {
+  if constexpr (OP != X_VMUL && OP != X_DENSITY) {
+    return false;
+  } else {
+#define QBH_X2(NN, A, B)                                                                  \
+  case NN:                                                                                  \
+    k_xpass2<T, OP, A, B><<<grid, NTHR, lds, ctx->stream>>>(g, t2, nslots, spw, in, out, dens, \
+                                                            cd);                            \
+    return true;
+  switch (g.np0) {
+    QBH_X2(28, 7, 4)
+    QBH_X2(36, 6, 6)
+    QBH_X2(72, 9, 8)
+    QBH_X2(140, 14, 10)
+    QBH_X2(144, 12, 12)
+    default:
+      return false;
+  }
+#undef QBH_X2
+  }
+}
+
 template <typename T, int OP>
 static int xpass_t(Ctx* ctx, Basis* b, const PipelineArgs& a, const double* in, double* out,
                    const double* dens, double scale, int cplx_density, const char* name) {
@@ -642,8 +851,10 @@ static int xpass_t(Ctx* ctx, Basis* b, const PipelineArgs& a, const double* in, 
   dim3 grid(g.np2 * nyb, (a.nslots + spw - 1) / spw);
   {
     TimedLaunch tl(ctx, name);
-    QBH_NMAX_DISPATCH(g.np0, (k_xpass<T, OP, NM><<<grid, NTHR, lds_x<T>(g), ctx->stream>>>(
-                                 g, t2, a.nslots, spw, in, out, dens, scale, cplx_density)));
+    if (!launch_x2<T, OP>(ctx, g, grid, lds_x<T>(g), t2, a.nslots, spw, in, out, dens,
+                          cplx_density))
+      QBH_NMAX_DISPATCH(g.np0, (k_xpass<T, OP, NM><<<grid, NTHR, lds_x<T>(g), ctx->stream>>>(
+                                   g, t2, a.nslots, spw, in, out, dens, scale, cplx_density)));
   }
   return check_launch(ctx, name);
 }
