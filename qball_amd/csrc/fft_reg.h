@@ -138,4 +138,57 @@ __device__ __forceinline__ void twiddle_row(cplx<T>* a, const cplx<T>* tw, int e
   }
 }
 
+// Wavefront-local transform of `nrows` (<= RPW) LDS rows of length n = N1*N2 at
+// buf + r*ld, natural order in and out, direction DIR (unnormalised):
+//   stage A: task (r, j2): DFT_N1 over x[j2 + N2*j1], twiddle W_n^{DIR j2 k1}, in place
+//   stage B: task (r, k1): DFT_N2 over the contiguous block N2*k1 + j2 -> X[k1 + N1*k2],
+//            held in registers until every lane has read its block, then stored at the
+//            natural positions.
+template <int N1, int N2, int RPW, int DIR, typename T>
+__device__ __forceinline__ void rows_fft(cplx<T>* buf, int nrows, int ld, const cplx<T>* tw,
+                                         int lane) {
+  constexpr int TA = RPW * N2, TB = RPW * N1;
+  constexpr int IA = (TA + 63) / 64, IB = (TB + 63) / 64;
+#pragma unroll
+  for (int ia = 0; ia < IA; ++ia) {
+    const int t = lane + 64 * ia;
+    const int r = t / N2, j2 = t % N2;
+    if (t < TA && r < nrows) {
+      cplx<T>* row = buf + r * ld + j2;
+      cplx<T> a[N1];
+#pragma unroll
+      for (int j1 = 0; j1 < N1; ++j1) a[j1] = row[N2 * j1];
+      sdft<N1, DIR>(a);
+      twiddle_row<N1, DIR>(a, tw, j2);
+#pragma unroll
+      for (int k1 = 0; k1 < N1; ++k1) row[N2 * k1] = a[k1];
+    }
+  }
+  wave_sync();
+  cplx<T> b[IB][N2];
+#pragma unroll
+  for (int ib = 0; ib < IB; ++ib) {
+    const int t = lane + 64 * ib;
+    const int r = t / N1, k1 = t % N1;
+    if (t < TB && r < nrows) {
+      const cplx<T>* blk = buf + r * ld + N2 * k1;
+#pragma unroll
+      for (int j2 = 0; j2 < N2; ++j2) b[ib][j2] = blk[j2];
+      sdft<N2, DIR>(b[ib]);
+    }
+  }
+  wave_sync();
+#pragma unroll
+  for (int ib = 0; ib < IB; ++ib) {
+    const int t = lane + 64 * ib;
+    const int r = t / N1, k1 = t % N1;
+    if (t < TB && r < nrows) {
+      cplx<T>* row = buf + r * ld + k1;
+#pragma unroll
+      for (int k2 = 0; k2 < N2; ++k2) row[N1 * k2] = b[ib][k2];
+    }
+  }
+  wave_sync();
+}
+
 }  // namespace qbh

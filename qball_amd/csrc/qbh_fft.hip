@@ -56,10 +56,21 @@ __device__ __forceinline__ int wrap(int l, int n) { return l < 0 ? l + n : l; }
 // elements per lane of a wavefront owning `lpw` lines of length <= NMAX
 template <int LPW, int NMAX> struct Epl { static constexpr int v = (LPW * NMAX + 63) / 64; };
 
+// Wavefront-local transform of nl lines of length ld: the two-factor register transform
+// when the length has an instantiated factorisation n = N1*N2, else the LDS Stockham passes.
+template <int DIR, int EPL, int N1, int N2, int RPW, typename T>
+__device__ __forceinline__ void line_fft(cplx<T>* buf, int nl, int ld, const AxisPlan& ax,
+                                         const cplx<T>* tw, int lane) {
+  if constexpr (N1 > 0)
+    rows_fft<N1, N2, RPW, DIR>(buf, nl, ld, tw, lane);
+  else
+    fft_lines_w<DIR, EPL>(buf, nl, ld, ax, tw, lane);
+}
+
 // ------------------------------------------------------------------------------------
 // z pass, backward: gather sphere coefficients of one slot into its columns, transform
 // along z (exp(+i)), store T1[slot][col][z].  Line e of a group belongs to wavefront e/4.
-template <typename T, int MODE, int NMAX>
+template <typename T, int MODE, int NMAX, int N1, int N2>
 __global__ void __launch_bounds__(NTHR)
 k_zpass_bwd(GridDev g, const double* __restrict__ psi, long ld, int nst, int band0,
             cplx<T>* __restrict__ t1) {
@@ -131,7 +142,7 @@ k_zpass_bwd(GridDev g, const double* __restrict__ psi, long ld, int nst, int ban
     }
   }
   wave_sync();
-  fft_lines_w<+1, EPL>(buf, nl, np2, g.ax[2], tw, lane);
+  line_fft<+1, EPL, N1, N2, ZLPW>(buf, nl, np2, g.ax[2], tw, lane);
   for (int ll = 0; ll < nl; ++ll) {
     const int col = g.ent_col[e0 + ll];
     if (col < 0) continue;
@@ -144,7 +155,7 @@ k_zpass_bwd(GridDev g, const double* __restrict__ psi, long ld, int nst, int ban
 // y pass, backward: T1 columns of ZB planes x LB x-lines -> y transform -> T2 lines.
 // The workgroup gathers its columns cooperatively (ZB-element segments), then each
 // wavefront transforms and stores its own LPW lines.
-template <typename T, int NMAX>
+template <typename T, int NMAX, int N1, int N2>
 __global__ void __launch_bounds__(NTHR)
 k_ypass_bwd(GridDev g, const cplx<T>* __restrict__ t1, cplx<T>* __restrict__ t2) {
   constexpr int ZB = Cfg<T>::ZB, LB = Cfg<T>::LB, LPW = Cfg<T>::LPW;
@@ -179,7 +190,7 @@ k_ypass_bwd(GridDev g, const cplx<T>* __restrict__ t1, cplx<T>* __restrict__ t2)
   __syncthreads();
   // line li = dz + ZB*dl; wavefront w owns lines [w*LPW, (w+1)*LPW)
   cplx<T>* wbuf = buf + wave * LPW * np1;
-  fft_lines_w<+1, EPL>(wbuf, LPW, np1, g.ax[1], tw, lane);
+  line_fft<+1, EPL, N1, N2, LPW>(wbuf, LPW, np1, g.ax[1], tw, lane);
   for (int ll = 0; ll < LPW; ++ll) {
     const int li = wave * LPW + ll;
     const int dz = li % ZB, dl = li / ZB;
@@ -191,7 +202,7 @@ k_ypass_bwd(GridDev g, const cplx<T>* __restrict__ t1, cplx<T>* __restrict__ t2)
 }
 
 // y pass, forward: T2 lines -> y transform (exp(-i)) -> sphere columns of T1.
-template <typename T, int NMAX>
+template <typename T, int NMAX, int N1, int N2>
 __global__ void __launch_bounds__(NTHR)
 k_ypass_fwd(GridDev g, const cplx<T>* __restrict__ t2, cplx<T>* __restrict__ t1) {
   constexpr int ZB = Cfg<T>::ZB, LB = Cfg<T>::LB, LPW = Cfg<T>::LPW;
@@ -220,7 +231,7 @@ k_ypass_fwd(GridDev g, const cplx<T>* __restrict__ t2, cplx<T>* __restrict__ t1)
     for (int y = lane; y < np1; y += 64) wbuf[ll * np1 + y] = ok ? src[y] : cplx<T>{T(0), T(0)};
   }
   wave_sync();
-  fft_lines_w<-1, EPL>(wbuf, LPW, np1, g.ax[1], tw, lane);
+  line_fft<-1, EPL, N1, N2, LPW>(wbuf, LPW, np1, g.ax[1], tw, lane);
   __syncthreads();
   const int c0 = g.line_start[lx0], c1 = g.line_start[lx1];
   const int nwork = (c1 - c0) * ZB;
@@ -587,7 +598,7 @@ k_xpass2(GridDev g, cplx<T>* __restrict__ t2, int nslots, int slots_per_wg,
 // ------------------------------------------------------------------------------------
 // z pass, forward: T1 columns -> z transform (exp(-i)) -> x 1/N -> unpack -> output.
 // out[n][G] = value (store) or out[n][G] = (out + kin[G]*psi[n][G]) + value.
-template <typename T, int MODE, int NMAX>
+template <typename T, int MODE, int NMAX, int N1, int N2>
 __global__ void __launch_bounds__(NTHR)
 k_zpass_fwd(GridDev g, const cplx<T>* __restrict__ t1, const double* __restrict__ psi, long ld,
             int nst, int band0, double* __restrict__ out, long ldo,
@@ -613,7 +624,7 @@ k_zpass_fwd(GridDev g, const cplx<T>* __restrict__ t1, const double* __restrict_
       buf[ll * np2 + z] = col < 0 ? cplx<T>{T(0), T(0)} : src[z];
   }
   wave_sync();
-  fft_lines_w<-1, EPL>(buf, nl, np2, g.ax[2], tw, lane);
+  line_fft<-1, EPL, N1, N2, ZLPW>(buf, nl, np2, g.ax[2], tw, lane);
 
   const double fac = 1.0 / ((double)g.np0 * (double)g.np1 * (double)g.np2);
   int n1, n2;
@@ -731,6 +742,38 @@ static int check_launch(Ctx* ctx, const char* what) {
     }                              \
   } while (0)
 
+// per-length dispatch for the z and y passes: F1*F2 = n selects the two-factor register
+// transform; F1 = 0 the LDS Stockham passes
+#define QBH_SIZE_DISPATCH(n, CALL)                       \
+  do {                                                   \
+    switch (n) {                                         \
+      case 36: {                                         \
+        constexpr int NM = 144, F1 = 6, F2 = 6;          \
+        CALL;                                            \
+      } break;                                           \
+      case 72: {                                         \
+        constexpr int NM = 144, F1 = 9, F2 = 8;          \
+        CALL;                                            \
+      } break;                                           \
+      case 140: {                                        \
+        constexpr int NM = 144, F1 = 14, F2 = 10;        \
+        CALL;                                            \
+      } break;                                           \
+      case 144: {                                        \
+        constexpr int NM = 144, F1 = 12, F2 = 12;        \
+        CALL;                                            \
+      } break;                                           \
+      default:                                           \
+        if ((n) <= 144) {                                \
+          constexpr int NM = 144, F1 = 0, F2 = 0;        \
+          CALL;                                          \
+        } else {                                         \
+          constexpr int NM = 256, F1 = 0, F2 = 0;        \
+          CALL;                                          \
+        }                                                \
+    }                                                    \
+  } while (0)
+
 template <typename T>
 static int bwd_yz_t(Ctx* ctx, Basis* b, const PipelineArgs& a) {
   const GridDev& g = b->g;
@@ -741,14 +784,14 @@ static int bwd_yz_t(Ctx* ctx, Basis* b, const PipelineArgs& a) {
     dim3 grid(g.ngroups, a.nslots);
     const size_t lds = lds_z<T>(g);
     if (a.mode == SLOT_PAIR)
-      QBH_NMAX_DISPATCH(g.np2, (k_zpass_bwd<T, SLOT_PAIR, NM><<<grid, NTHR, lds, ctx->stream>>>(
+      QBH_SIZE_DISPATCH(g.np2, (k_zpass_bwd<T, SLOT_PAIR, NM, F1, F2><<<grid, NTHR, lds, ctx->stream>>>(
                                    g, a.psi, a.ld, a.nst, a.band0, t1)));
     else if (a.mode == SLOT_SINGLE_REAL)
-      QBH_NMAX_DISPATCH(g.np2,
-                        (k_zpass_bwd<T, SLOT_SINGLE_REAL, NM><<<grid, NTHR, lds, ctx->stream>>>(
+      QBH_SIZE_DISPATCH(g.np2,
+                        (k_zpass_bwd<T, SLOT_SINGLE_REAL, NM, F1, F2><<<grid, NTHR, lds, ctx->stream>>>(
                             g, a.psi, a.ld, a.nst, a.band0, t1)));
     else
-      QBH_NMAX_DISPATCH(g.np2, (k_zpass_bwd<T, SLOT_COMPLEX, NM><<<grid, NTHR, lds, ctx->stream>>>(
+      QBH_SIZE_DISPATCH(g.np2, (k_zpass_bwd<T, SLOT_COMPLEX, NM, F1, F2><<<grid, NTHR, lds, ctx->stream>>>(
                                    g, a.psi, a.ld, a.nst, a.band0, t1)));
   }
   if (int r = check_launch(ctx, "zpass_bwd")) return r;
@@ -756,7 +799,7 @@ static int bwd_yz_t(Ctx* ctx, Basis* b, const PipelineArgs& a) {
     TimedLaunch tl(ctx, "ypass_bwd");
     constexpr int ZB = Cfg<T>::ZB, LB = Cfg<T>::LB;
     dim3 grid(((g.np2 + ZB - 1) / ZB) * ((g.nlx + LB - 1) / LB), a.nslots);
-    QBH_NMAX_DISPATCH(g.np1, (k_ypass_bwd<T, NM><<<grid, NTHR, lds_y<T>(g), ctx->stream>>>(
+    QBH_SIZE_DISPATCH(g.np1, (k_ypass_bwd<T, NM, F1, F2><<<grid, NTHR, lds_y<T>(g), ctx->stream>>>(
                                  g, t1, t2)));
   }
   return check_launch(ctx, "ypass_bwd");
@@ -776,7 +819,7 @@ static int fwd_zy_t(Ctx* ctx, Basis* b, const PipelineArgs& a, double* out, long
     TimedLaunch tl(ctx, "ypass_fwd");
     constexpr int ZB = Cfg<T>::ZB, LB = Cfg<T>::LB;
     dim3 grid(((g.np2 + ZB - 1) / ZB) * ((g.nlx + LB - 1) / LB), a.nslots);
-    QBH_NMAX_DISPATCH(g.np1, (k_ypass_fwd<T, NM><<<grid, NTHR, lds_y<T>(g), ctx->stream>>>(
+    QBH_SIZE_DISPATCH(g.np1, (k_ypass_fwd<T, NM, F1, F2><<<grid, NTHR, lds_y<T>(g), ctx->stream>>>(
                                  g, t2, t1)));
   }
   if (int r = check_launch(ctx, "ypass_fwd")) return r;
@@ -785,14 +828,14 @@ static int fwd_zy_t(Ctx* ctx, Basis* b, const PipelineArgs& a, double* out, long
     dim3 grid(g.ngroups, a.nslots);
     const size_t lds = lds_z<T>(g);
     if (a.mode == SLOT_PAIR)
-      QBH_NMAX_DISPATCH(g.np2, (k_zpass_fwd<T, SLOT_PAIR, NM><<<grid, NTHR, lds, ctx->stream>>>(
+      QBH_SIZE_DISPATCH(g.np2, (k_zpass_fwd<T, SLOT_PAIR, NM, F1, F2><<<grid, NTHR, lds, ctx->stream>>>(
                                    g, t1, a.psi, a.ld, a.nst, a.band0, out, ldo, kin, store)));
     else if (a.mode == SLOT_SINGLE_REAL)
-      QBH_NMAX_DISPATCH(g.np2,
-                        (k_zpass_fwd<T, SLOT_SINGLE_REAL, NM><<<grid, NTHR, lds, ctx->stream>>>(
+      QBH_SIZE_DISPATCH(g.np2,
+                        (k_zpass_fwd<T, SLOT_SINGLE_REAL, NM, F1, F2><<<grid, NTHR, lds, ctx->stream>>>(
                             g, t1, a.psi, a.ld, a.nst, a.band0, out, ldo, kin, store)));
     else
-      QBH_NMAX_DISPATCH(g.np2, (k_zpass_fwd<T, SLOT_COMPLEX, NM><<<grid, NTHR, lds, ctx->stream>>>(
+      QBH_SIZE_DISPATCH(g.np2, (k_zpass_fwd<T, SLOT_COMPLEX, NM, F1, F2><<<grid, NTHR, lds, ctx->stream>>>(
                                    g, t1, a.psi, a.ld, a.nst, a.band0, out, ldo, kin, store)));
   }
   return check_launch(ctx, "zpass_fwd");

@@ -256,3 +256,46 @@ def test_vector_potential_complex_si8(gpu_ctx, si8c):
     ham.close()
     basis.close()
     assert rel_err(got, ref) <= FP64_TOL
+
+
+@pytest.mark.parametrize("grid", [(40, 30, 42), (28, 36, 72), (30, 33, 44)])
+def test_noncubic_grids_generic_and_register_paths(gpu_ctx, si8, grid):
+    """Non-cubic grids mixing edges with and without a register factorisation: transforms
+    and local Hψ through both the LDS Stockham and the two-factor register passes."""
+    import torch
+    from oracle import fft as offt
+    from oracle import hpsi as oh
+    from oracle.basis import Basis as OB, FFTMaps, UnitCell
+    from qball_amd import Basis, Hamiltonian
+    ob = OB(UnitCell(si8.a), si8.ecut)
+    maps = FFTMaps(ob, *grid)
+    r = si8.rods
+    basis = Basis(gpu_ctx, True, r["rod_h"], r["rod_k"], r["rod_lmin"], r["rod_size"], grid, si8.b)
+    nb = si8.nst
+    cd = to_dev(si8.psi)
+    N = grid[0] * grid[1] * grid[2]
+    nslots = (nb + 1) // 2
+    f = torch.zeros(2 * nslots * N, dtype=torch.float64, device="cuda")
+    basis.backward(cd.data_ptr(), basis.ngw, nb, True, f.data_ptr())
+    gpu_ctx.sync()
+    fg = f.cpu().numpy().view(np.complex128).reshape(nslots, -1)
+    for s in range(nslots):
+        n2 = 2 * s + 1
+        ref = offt.backward(maps, si8.psi[:, 2 * s], si8.psi[:, n2] if n2 < nb else None).ravel()
+        assert rel_err(fg[s], ref) <= FP64_TOL
+    rng = np.random.default_rng(7)
+    v = rng.standard_normal(N)
+    vd = real_to_dev(v)
+    ham = Hamiltonian(gpu_ctx, basis)
+    ham.set_vloc(vd.data_ptr())
+    hd = torch.zeros_like(cd)
+    ham.hpsi(cd.data_ptr(), basis.ngw, nb, si8.occ, hd.data_ptr(), basis.ngw, flags=0x4)
+    gpu_ctx.sync()
+    ref = oh.rs_mul_add(maps, si8.psi, v, np.zeros_like(si8.psi))
+    assert rel_err(from_dev(hd, basis.ngw, nb), ref) <= FP64_TOL
+    rho = torch.zeros(N, dtype=torch.float64, device="cuda")
+    basis.density(cd.data_ptr(), basis.ngw, nb, si8.occ, 1.0, si8.omega, rho.data_ptr())
+    gpu_ctx.sync()
+    assert rel_err(rho.cpu().numpy(), oh.compute_density(maps, si8.psi, si8.occ, 1.0, si8.omega)) <= FP64_TOL
+    ham.close()
+    basis.close()
