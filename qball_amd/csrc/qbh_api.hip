@@ -300,6 +300,15 @@ extern "C" int qbh_ham_set_vector_potential(qbh_ham ham, const double* A) {
 }
 
 // ------------------------------------------------------------------------------------
+static int ensure_side(Ctx* c) {
+  if (c->side) return 0;
+  if (hipStreamCreate(&c->side) != hipSuccess ||
+      hipEventCreateWithFlags(&c->ev_fork, hipEventDisableTiming) != hipSuccess ||
+      hipEventCreateWithFlags(&c->ev_join, hipEventDisableTiming) != hipSuccess)
+    return fail(c, QBH_ERR_DEVICE, "side stream creation failed");
+  return 0;
+}
+
 static int hpsi_impl(Ham* h, const double* psi, long ld, int nst, const double* occ,
                      double* hpsi, long ldh, int flags, double* enl) {
   Ctx* c = h->ctx;
@@ -311,13 +320,37 @@ static int hpsi_impl(Ham* h, const double* psi, long ld, int nst, const double* 
   const int prec = fp32 ? PREC_FP32 : PREC_FP64;
   if (!(flags & QBH_ACCUMULATE)) fill0(c, hpsi, 2 * ldh * (long)nst);  // dwf.c().clear()
   if (enl) *enl = 0.0;
-  // non-local first (EnergyFunctional.cc:1500), then kinetic, then local (:1654-1695)
-  if ((flags & QBH_NONLOCAL) && h->P > 0) {
-    if (int r = nl_apply(h, psi, ld, nst, occ, hpsi, ldh, fp32 ? 1 : 0, enl)) return r;
-  }
+  const bool do_nl = (flags & QBH_NONLOCAL) && h->P > 0;
+  const bool do_loc = (flags & QBH_LOCAL) != 0;
   const bool kin = (flags & QBH_KINETIC) != 0;
-  if (flags & QBH_LOCAL) {
-    if (!h->v) return fail(c, QBH_ERR_ARG, "local potential not set");
+  if (do_loc && !h->v) return fail(c, QBH_ERR_ARG, "local potential not set");
+  hipStream_t main = c->stream;
+  // Order of accumulation into hpsi: non-local (EnergyFunctional.cc:1500), then kinetic,
+  // then local (:1654-1695).  The projector GEMMs run on the side stream, concurrently
+  // with the backward/x/forward-y transform passes; the forward z pass, which adds the
+  // kinetic and local terms to hpsi, waits for them.
+  hipEvent_t join = nullptr;
+  if (do_nl) {
+    const bool overlap = do_loc;
+    if (overlap) {
+      if (int r = ensure_side(c)) return r;
+      hipEventRecord(c->ev_fork, main);
+      hipStreamWaitEvent(c->side, c->ev_fork, 0);
+      c->stream = c->side;
+    }
+    double dummy = 0.0;
+    const int r = nl_apply(h, psi, ld, nst, occ, hpsi, ldh, fp32 ? 1 : 0, enl ? &dummy : nullptr);
+    if (overlap) {
+      hipEventRecord(c->ev_join, c->side);
+      c->stream = main;
+      join = c->ev_join;
+    }
+    if (r) {
+      if (join) hipStreamWaitEvent(main, join, 0);
+      return r;
+    }
+  }
+  if (do_loc) {
     const int mode = b->g.real ? SLOT_PAIR : SLOT_COMPLEX;
     const int nslots = b->g.real ? (nst + 1) / 2 : nst;
     int r = for_chunks(c, b, nslots, prec, [&](PipelineArgs& a, int s0) {
@@ -326,10 +359,12 @@ static int hpsi_impl(Ham* h, const double* psi, long ld, int nst, const double* 
       a.nst = nst;
       a.band0 = b->g.real ? 2 * s0 : s0;
       a.mode = mode;
+      a.wait_z = join;
       if (int r = pipeline_bwd_yz(c, b, a)) return r;
       if (int r = pipeline_x_vmul(c, b, a, h->v)) return r;
       return pipeline_fwd_zy_accumulate(c, b, a, hpsi, ldh, kin ? h->kin : nullptr, 0);
     });
+    if (join) hipStreamWaitEvent(main, join, 0);  // also on early exit
     if (r) return r;
   } else if (kin) {
     TimedLaunch tl(c, "kinetic");
@@ -337,7 +372,9 @@ static int hpsi_impl(Ham* h, const double* psi, long ld, int nst, const double* 
     k_kinetic<<<(unsigned)((n + 255) / 256), 256, 0, c->stream>>>(ngw, nst, h->kin, psi, ld,
                                                                   hpsi, ldh);
   }
-  return dev_check(c, "hpsi");
+  if (int r = dev_check(c, "hpsi")) return r;
+  if (enl && do_nl) *enl = nl_read_enl(h, main);
+  return 0;
 }
 
 extern "C" int qbh_hpsi(qbh_ham ham, const double* psi_dev, long ld, int nst, const double* occ,

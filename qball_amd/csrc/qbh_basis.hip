@@ -95,6 +95,12 @@ extern "C" int qbh_ctx_destroy(qbh_ctx ctx) {
     hipEventDestroy(r.b);
   }
   qbh::blas_destroy(&ctx->c);
+  if (ctx->c.side) {
+    hipStreamSynchronize(ctx->c.side);
+    hipStreamDestroy(ctx->c.side);
+  }
+  if (ctx->c.ev_fork) hipEventDestroy(ctx->c.ev_fork);
+  if (ctx->c.ev_join) hipEventDestroy(ctx->c.ev_join);
   if (ctx->c.ws) hipFree(ctx->c.ws);
   if (ctx->c.own_stream) hipStreamDestroy(ctx->c.stream);
   delete ctx;

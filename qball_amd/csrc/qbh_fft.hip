@@ -824,6 +824,7 @@ static int fwd_zy_t(Ctx* ctx, Basis* b, const PipelineArgs& a, double* out, long
   }
   if (int r = check_launch(ctx, "ypass_fwd")) return r;
   {
+    if (a.wait_z) hipStreamWaitEvent(ctx->stream, a.wait_z, 0);
     TimedLaunch tl(ctx, "zpass_fwd");
     dim3 grid(g.ngroups, a.nslots);
     const size_t lds = lds_z<T>(g);

@@ -44,6 +44,7 @@ struct Ham {
 
 int nl_apply(Ham* h, const double* psi, long ld, int nst, const double* occ, double* hpsi,
              long ldh, int fp32, double* enl);
+double nl_read_enl(Ham* h, hipStream_t s);
 int blas_handle(Ctx* c, void** out);
 void* grow(void*& p, size_t& have, size_t need);
 

@@ -84,6 +84,10 @@ struct Ctx {
   void* blas = nullptr;  // rocblas_handle
   std::string last_error;
   KernelTimer timer;
+  // second stream (default priority) for the projector GEMMs, forked from and joined to
+  // `stream` with events so they overlap the transform pipeline
+  hipStream_t side = nullptr;
+  hipEvent_t ev_fork = nullptr, ev_join = nullptr;
   // workspace (grown on demand, reused across calls)
   void* ws = nullptr;
   size_t ws_bytes = 0;
@@ -115,6 +119,7 @@ struct PipelineArgs {
   int prec;            // Prec
   void* t1;            // workspace
   void* t2;
+  hipEvent_t wait_z = nullptr;  // the forward z pass (which writes the output) waits on it
 };
 
 int pipeline_bwd_yz(Ctx* ctx, Basis* b, const PipelineArgs& a);

@@ -415,13 +415,19 @@ int nl_apply(Ham* h, const double* psi, long ld, int nst, const double* occ_host
     return QBH_ERR_DEVICE;
   }
   if (enl) {
+    // Enl is reduced on the device; nl_read_enl fetches it once the stream has drained
     k_sum_partials<<<1, 256, 0, c->stream>>>(nblk, h->red);
-    double v = 0.0;
-    hipMemcpyAsync(&v, h->red + nblk, sizeof(double), hipMemcpyDeviceToHost, c->stream);
-    hipStreamSynchronize(c->stream);
-    *enl = v;
+    *enl = 0.0;
   }
   return 0;
+}
+
+double nl_read_enl(Ham* h, hipStream_t s) {
+  double v = 0.0;
+  if (!h->red || h->P == 0) return 0.0;
+  hipMemcpyAsync(&v, h->red + 512, sizeof(double), hipMemcpyDeviceToHost, s);
+  hipStreamSynchronize(s);
+  return v;
 }
 
 }  // namespace qbh
