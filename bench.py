@@ -93,6 +93,28 @@ def main():
     for _ in range(args.warmup):
         step(QBH_ALL)
     torch.cuda.synchronize()
+
+    # schedule choice (untimed probe): projector GEMMs concurrent with the transforms on
+    # a second stream, or after them on one stream
+    def probe(flags, n=3):
+        torch.cuda.synchronize()
+        t = time.perf_counter()
+        for _ in range(n):
+            step(flags)
+        torch.cuda.synchronize()
+        return (time.perf_counter() - t) / n
+
+    def pick_schedule(flags):
+        ctx.set_overlap(True)
+        t_ov = probe(flags)
+        ctx.set_overlap(False)
+        t_seq = probe(flags)
+        ov = t_ov <= t_seq
+        ctx.set_overlap(ov)
+        return {"overlap_ms": t_ov * 1e3, "sequential_ms": t_seq * 1e3,
+                "chosen": "overlap" if ov else "sequential"}
+
+    sched64 = pick_schedule(QBH_ALL)
     ctx.timing_report()                        # discard warm-up totals
     ctx.set_timing(True)
     torch.cuda.synchronize()
@@ -148,6 +170,7 @@ def main():
         h64 = hpsi.clone()
         for _ in range(args.warmup):
             step(QBH_ALL | QBH_FP32)
+        sched32 = pick_schedule(QBH_ALL | QBH_FP32)
         torch.cuda.synchronize()
         ctx.timing_report()
         ctx.set_timing(True)
@@ -164,7 +187,7 @@ def main():
         fp32 = {"value": nst / dt32, "ms_per_step": dt32 * 1e3, "speedup_vs_fp64": dt / dt32,
                 "hpsi_rel_err_vs_fp64": rel,
                 "band_energy_diff_per_atom_Ha": abs(e32 - e64) / sysm.na,
-                "kernels": k32}
+                "schedule": sched32, "kernels": k32}
         del h64
 
     # CPU baseline: the oracle (numpy restatement) on a bounded sample, 1 thread
@@ -191,6 +214,7 @@ def main():
         "roofline": roof,
         "cpu_baseline": cpu,
         "fp32_bulk": fp32,
+        "schedule": sched64,
         "kernels": per_kernel,
         "band_energy_Ha": e64,
         "setup_s": t_setup,

@@ -50,6 +50,10 @@ int qbh_sync(qbh_ctx ctx);
 int qbh_ctx_set_workspace_cap(qbh_ctx ctx, double bytes);
 /* Per-kernel timing with HIP events on the context's stream (0 = off). */
 int qbh_ctx_set_timing(qbh_ctx ctx, int on);
+/* Run the projector GEMMs of qbh_hpsi on a second stream, concurrently with the local-
+ * potential transforms (1, default), or after them on the context's stream (0).  Results
+ * are identical either way. */
+int qbh_ctx_set_overlap(qbh_ctx ctx, int on);
 /* Copies up to `cap` "name total_ms count" lines (newline separated) into buf and
  * resets the accumulated totals.  Synchronises the stream. */
 int qbh_ctx_timing_report(qbh_ctx ctx, char* buf, int cap);

@@ -30,6 +30,7 @@ SIGNATURES = {
     "qbh_sync": (_c_int, [_vp]),
     "qbh_ctx_set_workspace_cap": (_c_int, [_vp, _c_double]),
     "qbh_ctx_set_timing": (_c_int, [_vp, _c_int]),
+    "qbh_ctx_set_overlap": (_c_int, [_vp, _c_int]),
     "qbh_ctx_timing_report": (_c_int, [_vp, ctypes.c_char_p, _c_int]),
     "qbh_basis_create": (_c_int, [_vp, _c_int, _c_int, _ip, _ip, _ip, _ip, _c_int, _c_int,
                                   _c_int, _dp, _dp, ctypes.POINTER(_vp)]),
@@ -127,6 +128,9 @@ class Context:
 
     def set_workspace_cap(self, nbytes):
         self.check(lib().qbh_ctx_set_workspace_cap(self.h, float(nbytes)), "workspace cap")
+
+    def set_overlap(self, on):
+        self.check(lib().qbh_ctx_set_overlap(self.h, int(bool(on))), "set_overlap")
 
     def set_timing(self, on):
         self.check(lib().qbh_ctx_set_timing(self.h, int(bool(on))), "set_timing")

@@ -331,7 +331,7 @@ static int hpsi_impl(Ham* h, const double* psi, long ld, int nst, const double* 
   // kinetic and local terms to hpsi, waits for them.
   hipEvent_t join = nullptr;
   if (do_nl) {
-    const bool overlap = do_loc;
+    const bool overlap = do_loc && c->overlap;
     if (overlap) {
       if (int r = ensure_side(c)) return r;
       hipEventRecord(c->ev_fork, main);

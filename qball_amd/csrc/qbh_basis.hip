@@ -124,6 +124,12 @@ extern "C" int qbh_ctx_set_workspace_cap(qbh_ctx ctx, double bytes) {
   return QBH_OK;
 }
 
+extern "C" int qbh_ctx_set_overlap(qbh_ctx ctx, int on) {
+  if (!ctx) return QBH_ERR_ARG;
+  ctx->c.overlap = on != 0;
+  return QBH_OK;
+}
+
 extern "C" int qbh_ctx_set_timing(qbh_ctx ctx, int on) {
   if (!ctx) return QBH_ERR_ARG;
   ctx->c.timer.enabled = on != 0;
@@ -303,6 +309,12 @@ extern "C" int qbh_basis_create(qbh_ctx ctx, int real, int nrods, const int* rod
   }
   line_start.push_back(ncol);
   g.nlx = (int)line_x.size();
+  g.xpos_max = -1;
+  g.xneg_min = np0;
+  for (int x : line_x) {
+    if (2 * x < np0) g.xpos_max = std::max(g.xpos_max, x);
+    else g.xneg_min = std::min(g.xneg_min, x);
+  }
 
   // z-pass groups: consecutive columns in creation order (the +G/-G images of a rod
   // are adjacent), at most kZL lines per group (4 per wavefront)

@@ -476,9 +476,20 @@ k_xpass2(GridDev g, cplx<T>* __restrict__ t2, int nslots, int slots_per_wg,
   };
   if (s0 < s1) prefetch(s0);
 
+  // The sphere lines occupy x in [0, xpos_max] and [xneg_min, n).  Stage A input index
+  // j1 in [ja, jb] is zero for every j2 and is not read; stage A' outputs there are not
+  // needed.  Only the rest of the rows outside the lines has to be cleared per slot.
+  const int ja = g.xpos_max / N2 + 1;
+  const int jb = g.xneg_min / N2 - 1;
+  const int z0a = g.xpos_max + 1, z0b = min(N2 * ja, g.xneg_min);
+  const int z1a = max(N2 * (jb + 1), z0b), z1b = g.xneg_min;
+  const int nz0 = max(0, z0b - z0a), nz1 = max(0, z1b - z1a);
+
   for (int s = s0; s < s1; ++s) {
-    for (int i = lane; i < LPW * n; i += 64) buf[i] = {T(0), T(0)};
-    wave_sync();
+    for (int i = lane; i < LPW * (nz0 + nz1); i += 64) {
+      const int r = i / (nz0 + nz1), k = i % (nz0 + nz1);
+      buf[r * n + (k < nz0 ? z0a + k : z1a + k - nz0)] = {T(0), T(0)};
+    }
 #pragma unroll
     for (int k = 0; k < GPL; ++k) {
       const int q = lane + k * 64;
@@ -496,7 +507,8 @@ k_xpass2(GridDev g, cplx<T>* __restrict__ t2, int nslots, int slots_per_wg,
         cplx<T>* row = buf + r * n + j2;
         cplx<T> a[N1];
 #pragma unroll
-        for (int j1 = 0; j1 < N1; ++j1) a[j1] = row[N2 * j1];
+        for (int j1 = 0; j1 < N1; ++j1)
+          a[j1] = (j1 < ja || j1 > jb) ? row[N2 * j1] : cplx<T>{T(0), T(0)};
         sdft<N1, +1>(a);
         twiddle_row<N1, +1>(a, tw, j2);
 #pragma unroll
@@ -571,7 +583,8 @@ k_xpass2(GridDev g, cplx<T>* __restrict__ t2, int nslots, int slots_per_wg,
         for (int k1 = 0; k1 < N1; ++k1) a[k1] = row[N2 * k1];
         sdft<N1, -1>(a);
 #pragma unroll
-        for (int m1 = 0; m1 < N1; ++m1) row[N2 * m1] = a[m1];
+        for (int m1 = 0; m1 < N1; ++m1)
+          if (m1 < ja || m1 > jb) row[N2 * m1] = a[m1];
       }
     }
     wave_sync();

@@ -29,6 +29,8 @@ struct GridDev {
   int nlx;      // distinct x positions among the columns
   int real;     // Γ half-sphere storage
   int ngroups;  // rod groups for the z passes
+  int xpos_max; // largest x position of a line with h >= 0 (lines cover [0, xpos_max])
+  int xneg_min; // smallest x position of a line with h < 0 (lines cover [xneg_min, np0))
   // rods (caller's order)
   const int* rod_first;
   const int* rod_lmin;
@@ -88,6 +90,7 @@ struct Ctx {
   // `stream` with events so they overlap the transform pipeline
   hipStream_t side = nullptr;
   hipEvent_t ev_fork = nullptr, ev_join = nullptr;
+  bool overlap = true;  // run the projector GEMMs concurrently with the transforms
   // workspace (grown on demand, reused across calls)
   void* ws = nullptr;
   size_t ws_bytes = 0;
