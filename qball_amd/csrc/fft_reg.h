@@ -138,8 +138,24 @@ __device__ __forceinline__ void twiddle_row(cplx<T>* a, const cplx<T>* tw, int e
   }
 }
 
+// Padded LDS row layout for the two-factor transform: element x of a row lives at
+// x + x / N2 when N2 is even, so the stage-B lanes (stride N2 + 1 elements, odd) hit
+// distinct LDS banks; the row occupies N1 * (N2 + 1) elements.  N1 = 0 (no register
+// factorisation) means an unpadded row.
+// Off by default (QBH_LDS_PAD=0): measured on MI355X the padded rows raise the LDS
+// footprint of the y/z passes from 38 to 42 KB per workgroup (4 -> 3 workgroups per CU)
+// and cost more than the 2-way stage-B conflicts they remove.
+#ifndef QBH_LDS_PAD
+#define QBH_LDS_PAD 0
+#endif
+template <int N1, int N2> struct Pad {
+  static constexpr int P = (QBH_LDS_PAD && N1 > 0 && N2 % 2 == 0) ? 1 : 0;
+  static __host__ __device__ constexpr int idx(int x) { return P ? x + x / N2 : x; }
+  static __host__ __device__ constexpr int row(int n) { return P ? n + n / N2 : n; }
+};
+
 // Wavefront-local transform of `nrows` (<= RPW) LDS rows of length n = N1*N2 at
-// buf + r*ld, natural order in and out, direction DIR (unnormalised):
+// buf + r*ld (padded layout, Pad<N1,N2>), natural order in and out, direction DIR:
 //   stage A: task (r, j2): DFT_N1 over x[j2 + N2*j1], twiddle W_n^{DIR j2 k1}, in place
 //   stage B: task (r, k1): DFT_N2 over the contiguous block N2*k1 + j2 -> X[k1 + N1*k2],
 //            held in registers until every lane has read its block, then stored at the
@@ -149,6 +165,7 @@ __device__ __forceinline__ void rows_fft(cplx<T>* buf, int nrows, int ld, const 
                                          int lane) {
   constexpr int TA = RPW * N2, TB = RPW * N1;
   constexpr int IA = (TA + 63) / 64, IB = (TB + 63) / 64;
+  constexpr int S = N2 + Pad<N1, N2>::P;  // padded stride of the N2-blocks
 #pragma unroll
   for (int ia = 0; ia < IA; ++ia) {
     const int t = lane + 64 * ia;
@@ -157,11 +174,11 @@ __device__ __forceinline__ void rows_fft(cplx<T>* buf, int nrows, int ld, const 
       cplx<T>* row = buf + r * ld + j2;
       cplx<T> a[N1];
 #pragma unroll
-      for (int j1 = 0; j1 < N1; ++j1) a[j1] = row[N2 * j1];
+      for (int j1 = 0; j1 < N1; ++j1) a[j1] = row[S * j1];
       sdft<N1, DIR>(a);
       twiddle_row<N1, DIR>(a, tw, j2);
 #pragma unroll
-      for (int k1 = 0; k1 < N1; ++k1) row[N2 * k1] = a[k1];
+      for (int k1 = 0; k1 < N1; ++k1) row[S * k1] = a[k1];
     }
   }
   wave_sync();
@@ -171,7 +188,7 @@ __device__ __forceinline__ void rows_fft(cplx<T>* buf, int nrows, int ld, const 
     const int t = lane + 64 * ib;
     const int r = t / N1, k1 = t % N1;
     if (t < TB && r < nrows) {
-      const cplx<T>* blk = buf + r * ld + N2 * k1;
+      const cplx<T>* blk = buf + r * ld + S * k1;
 #pragma unroll
       for (int j2 = 0; j2 < N2; ++j2) b[ib][j2] = blk[j2];
       sdft<N2, DIR>(b[ib]);
@@ -183,9 +200,9 @@ __device__ __forceinline__ void rows_fft(cplx<T>* buf, int nrows, int ld, const 
     const int t = lane + 64 * ib;
     const int r = t / N1, k1 = t % N1;
     if (t < TB && r < nrows) {
-      cplx<T>* row = buf + r * ld + k1;
+      cplx<T>* row = buf + r * ld;
 #pragma unroll
-      for (int k2 = 0; k2 < N2; ++k2) row[N1 * k2] = b[ib][k2];
+      for (int k2 = 0; k2 < N2; ++k2) row[Pad<N1, N2>::idx(k1 + N1 * k2)] = b[ib][k2];
     }
   }
   wave_sync();

@@ -29,12 +29,27 @@ constexpr int ZLPW = 4;    // z-pass lines per wavefront (16 per group)
 
 // per precision: lines per wavefront in the y and x passes, y-pass plane block
 template <typename T> struct Cfg;
+// x-pass rows per wavefront and launch tuning (compile-time knobs)
+#ifndef QBH_XR64
+#define QBH_XR64 4
+#endif
+#ifndef QBH_XR32
+#define QBH_XR32 4
+#endif
+#ifndef QBH_X_PREFETCH
+#define QBH_X_PREFETCH 1
+#endif
+#ifndef QBH_XWPE
+#define QBH_XWPE 0
+#endif
 template <> struct Cfg<double> {
-  static constexpr int LPW = 4;   // 4 lines x 16 B x ... : 64 B row segments in the x pass
+  static constexpr int XR = QBH_XR64;  // x pass: rows per wavefront (64 B row segments)
+  static constexpr int LPW = 4;
   static constexpr int ZB = 8;    // y pass: 8 planes x 16 B = 128 B column segments
   static constexpr int LB = 2;    // y pass: x-lines per workgroup (ZB*LB = 4*LPW)
 };
 template <> struct Cfg<float> {
+  static constexpr int XR = QBH_XR32;
   static constexpr int LPW = 8;
   static constexpr int ZB = 16;
   static constexpr int LB = 2;
@@ -87,8 +102,10 @@ k_zpass_bwd(GridDev g, const double* __restrict__ psi, long ld, int nst, int ban
   const int e0 = g.grp_start[grp] + wave * ZLPW;
   const int nl = min(ZLPW, g.grp_start[grp + 1] - e0);
   if (nl <= 0) return;
-  cplx<T>* buf = tw + np2 + wave * ZLPW * np2;
-  for (int i = lane; i < nl * np2; i += 64) buf[i] = {T(0), T(0)};
+  using PD = Pad<N1, N2>;
+  const int LD = PD::row(np2);
+  cplx<T>* buf = tw + np2 + wave * ZLPW * LD;
+  for (int i = lane; i < nl * LD; i += 64) buf[i] = {T(0), T(0)};
   wave_sync();
 
   int n1, n2;
@@ -114,7 +131,7 @@ k_zpass_bwd(GridDev g, const double* __restrict__ psi, long ld, int nst, int ban
       const int first = g.rod_first[rod];
       const int lmin = g.rod_lmin[rod];
       const int size = g.rod_size[rod];
-      cplx<T>* line = buf + le * np2;
+      cplx<T>* line = buf + le * LD;
       for (int i = sub; i < size; i += 16) {
         const int l = lmin + i;
         const double a = p1[2 * (first + i)];
@@ -131,23 +148,23 @@ k_zpass_bwd(GridDev g, const double* __restrict__ psi, long ld, int nst, int ban
           pr = a; pi = b; mr = a; mi = -b;
         }
         if (kind == 0) {
-          if (i > 0) line[wrap(l, np2)] = {T(pr), T(pi)};
-          line[wrap(-l, np2)] = {T(mr), T(mi)};
+          if (i > 0) line[PD::idx(wrap(l, np2))] = {T(pr), T(pi)};
+          line[PD::idx(wrap(-l, np2))] = {T(mr), T(mi)};
         } else if (kind == 1 || kind == 3) {
-          line[wrap(l, np2)] = {T(pr), T(pi)};
+          line[PD::idx(wrap(l, np2))] = {T(pr), T(pi)};
         } else {
-          line[wrap(-l, np2)] = {T(mr), T(mi)};
+          line[PD::idx(wrap(-l, np2))] = {T(mr), T(mi)};
         }
       }
     }
   }
   wave_sync();
-  line_fft<+1, EPL, N1, N2, ZLPW>(buf, nl, np2, g.ax[2], tw, lane);
+  line_fft<+1, EPL, N1, N2, ZLPW>(buf, nl, LD, g.ax[2], tw, lane);
   for (int ll = 0; ll < nl; ++ll) {
     const int col = g.ent_col[e0 + ll];
     if (col < 0) continue;
     cplx<T>* dst = t1 + ((long)slot * g.ncol + col) * np2;
-    for (int z = lane; z < np2; z += 64) dst[z] = buf[ll * np2 + z];
+    for (int z = lane; z < np2; z += 64) dst[z] = buf[ll * LD + PD::idx(z)];
   }
 }
 
@@ -164,6 +181,8 @@ k_ypass_bwd(GridDev g, const cplx<T>* __restrict__ t1, cplx<T>* __restrict__ t2)
   const int np1 = g.np1, np2 = g.np2, nlx = g.nlx;
   cplx<T>* tw = reinterpret_cast<cplx<T>*>(smem_raw);
   cplx<T>* buf = tw + np1;
+  using PD = Pad<N1, N2>;
+  const int LD = PD::row(np1);
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
   const int nzb = (np2 + ZB - 1) / ZB;
   const int zb = blockIdx.x % nzb;
@@ -173,7 +192,7 @@ k_ypass_bwd(GridDev g, const cplx<T>* __restrict__ t1, cplx<T>* __restrict__ t2)
   const int lx1 = min(lx0 + LB, nlx);
   const cplx<T>* twg = twtab<T>(g, 1);
   for (int i = threadIdx.x; i < np1; i += NTHR) tw[i] = twg[i];
-  for (int i = threadIdx.x; i < ZB * LB * np1; i += NTHR) buf[i] = {T(0), T(0)};
+  for (int i = threadIdx.x; i < ZB * LB * LD; i += NTHR) buf[i] = {T(0), T(0)};
   __syncthreads();
   const int c0 = g.line_start[lx0], c1 = g.line_start[lx1];
   const int nwork = (c1 - c0) * ZB;
@@ -184,20 +203,20 @@ k_ypass_bwd(GridDev g, const cplx<T>* __restrict__ t1, cplx<T>* __restrict__ t2)
     if (z < np2) {
       int dl = 0;
       while (lx0 + dl + 1 < lx1 && c >= g.line_start[lx0 + dl + 1]) ++dl;
-      buf[(dz + ZB * dl) * np1 + g.col_y[c]] = t1[((long)slot * g.ncol + c) * np2 + z];
+      buf[(dz + ZB * dl) * LD + PD::idx(g.col_y[c])] = t1[((long)slot * g.ncol + c) * np2 + z];
     }
   }
   __syncthreads();
   // line li = dz + ZB*dl; wavefront w owns lines [w*LPW, (w+1)*LPW)
-  cplx<T>* wbuf = buf + wave * LPW * np1;
-  line_fft<+1, EPL, N1, N2, LPW>(wbuf, LPW, np1, g.ax[1], tw, lane);
+  cplx<T>* wbuf = buf + wave * LPW * LD;
+  line_fft<+1, EPL, N1, N2, LPW>(wbuf, LPW, LD, g.ax[1], tw, lane);
   for (int ll = 0; ll < LPW; ++ll) {
     const int li = wave * LPW + ll;
     const int dz = li % ZB, dl = li / ZB;
     const int z = z0 + dz, lx = lx0 + dl;
     if (z >= np2 || lx >= lx1) continue;
     cplx<T>* dst = t2 + (((long)slot * np2 + z) * nlx + lx) * np1;
-    for (int y = lane; y < np1; y += 64) dst[y] = wbuf[ll * np1 + y];
+    for (int y = lane; y < np1; y += 64) dst[y] = wbuf[ll * LD + PD::idx(y)];
   }
 }
 
@@ -211,6 +230,8 @@ k_ypass_fwd(GridDev g, const cplx<T>* __restrict__ t2, cplx<T>* __restrict__ t1)
   const int np1 = g.np1, np2 = g.np2, nlx = g.nlx;
   cplx<T>* tw = reinterpret_cast<cplx<T>*>(smem_raw);
   cplx<T>* buf = tw + np1;
+  using PD = Pad<N1, N2>;
+  const int LD = PD::row(np1);
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
   const int nzb = (np2 + ZB - 1) / ZB;
   const int zb = blockIdx.x % nzb;
@@ -221,17 +242,18 @@ k_ypass_fwd(GridDev g, const cplx<T>* __restrict__ t2, cplx<T>* __restrict__ t1)
   const cplx<T>* twg = twtab<T>(g, 1);
   for (int i = threadIdx.x; i < np1; i += NTHR) tw[i] = twg[i];
   __syncthreads();
-  cplx<T>* wbuf = buf + wave * LPW * np1;
+  cplx<T>* wbuf = buf + wave * LPW * LD;
   for (int ll = 0; ll < LPW; ++ll) {
     const int li = wave * LPW + ll;
     const int dz = li % ZB, dl = li / ZB;
     const int z = z0 + dz, lx = lx0 + dl;
     const bool ok = z < np2 && lx < lx1;
     const cplx<T>* src = t2 + (((long)slot * np2 + (ok ? z : 0)) * nlx + (ok ? lx : 0)) * np1;
-    for (int y = lane; y < np1; y += 64) wbuf[ll * np1 + y] = ok ? src[y] : cplx<T>{T(0), T(0)};
+    for (int y = lane; y < np1; y += 64)
+      wbuf[ll * LD + PD::idx(y)] = ok ? src[y] : cplx<T>{T(0), T(0)};
   }
   wave_sync();
-  line_fft<-1, EPL, N1, N2, LPW>(wbuf, LPW, np1, g.ax[1], tw, lane);
+  line_fft<-1, EPL, N1, N2, LPW>(wbuf, LPW, LD, g.ax[1], tw, lane);
   __syncthreads();
   const int c0 = g.line_start[lx0], c1 = g.line_start[lx1];
   const int nwork = (c1 - c0) * ZB;
@@ -242,7 +264,7 @@ k_ypass_fwd(GridDev g, const cplx<T>* __restrict__ t2, cplx<T>* __restrict__ t1)
     if (z < np2) {
       int dl = 0;
       while (lx0 + dl + 1 < lx1 && c >= g.line_start[lx0 + dl + 1]) ++dl;
-      t1[((long)slot * g.ncol + c) * np2 + z] = buf[(dz + ZB * dl) * np1 + g.col_y[c]];
+      t1[((long)slot * g.ncol + c) * np2 + z] = buf[(dz + ZB * dl) * LD + PD::idx(g.col_y[c])];
     }
   }
 }
@@ -424,10 +446,13 @@ k_xpass(GridDev g, cplx<T>* __restrict__ t2, int nslots, int slots_per_wg,
 // multiply touches each element once, in registers.
 template <typename T, int OP, int N1, int N2>
 __global__ void __launch_bounds__(NTHR)
+#if QBH_XWPE > 0
+__attribute__((amdgpu_waves_per_eu(QBH_XWPE)))
+#endif
 k_xpass2(GridDev g, cplx<T>* __restrict__ t2, int nslots, int slots_per_wg,
          const double* __restrict__ v, double* __restrict__ out, const double* __restrict__ dens,
          int cplx_density) {
-  constexpr int LPW = Cfg<T>::LPW;
+  constexpr int LPW = Cfg<T>::XR;
   constexpr int YB = LPW * NWAVE;
   constexpr int n = N1 * N2;
   constexpr int TA = LPW * N2, TB = LPW * N1;
@@ -447,7 +472,10 @@ k_xpass2(GridDev g, cplx<T>* __restrict__ t2, int nslots, int slots_per_wg,
   for (int i = threadIdx.x; i < n; i += NTHR) tw[i] = twg[i];
   __syncthreads();
   if (nyl <= 0) return;
-  cplx<T>* buf = tw + n + wave * LPW * n;
+  using PD = Pad<N1, N2>;
+  constexpr int LD = PD::row(n);
+  constexpr int S = N2 + PD::P;
+  cplx<T>* buf = tw + n + wave * LPW * LD;
   const long grow0 = ((long)z * np1 + y0) * n;
   const int ng = nlx * LPW;
 
@@ -474,7 +502,7 @@ k_xpass2(GridDev g, cplx<T>* __restrict__ t2, int nslots, int slots_per_wg,
                                    : cplx<T>{T(0), T(0)};
     }
   };
-  if (s0 < s1) prefetch(s0);
+  if (QBH_X_PREFETCH && s0 < s1) prefetch(s0);
 
   // The sphere lines occupy x in [0, xpos_max] and [xneg_min, n).  Stage A input index
   // j1 in [ja, jb] is zero for every j2 and is not read; stage A' outputs there are not
@@ -488,15 +516,16 @@ k_xpass2(GridDev g, cplx<T>* __restrict__ t2, int nslots, int slots_per_wg,
   for (int s = s0; s < s1; ++s) {
     for (int i = lane; i < LPW * (nz0 + nz1); i += 64) {
       const int r = i / (nz0 + nz1), k = i % (nz0 + nz1);
-      buf[r * n + (k < nz0 ? z0a + k : z1a + k - nz0)] = {T(0), T(0)};
+      buf[r * LD + PD::idx(k < nz0 ? z0a + k : z1a + k - nz0)] = {T(0), T(0)};
     }
+    if (!QBH_X_PREFETCH) prefetch(s);
 #pragma unroll
     for (int k = 0; k < GPL; ++k) {
       const int q = lane + k * 64;
-      if (q < ng) buf[(q % LPW) * n + g.line_x[q / LPW]] = pf[k];
+      if (q < ng) buf[(q % LPW) * LD + PD::idx(g.line_x[q / LPW])] = pf[k];
     }
     wave_sync();
-    if (s + 1 < s1) prefetch(s + 1);
+    if (QBH_X_PREFETCH && s + 1 < s1) prefetch(s + 1);
 
     // stage A, backward
 #pragma unroll
@@ -504,15 +533,15 @@ k_xpass2(GridDev g, cplx<T>* __restrict__ t2, int nslots, int slots_per_wg,
       const int t = lane + 64 * ia;
       if (t < TA) {
         const int r = t / N2, j2 = t % N2;
-        cplx<T>* row = buf + r * n + j2;
+        cplx<T>* row = buf + r * LD + j2;
         cplx<T> a[N1];
 #pragma unroll
         for (int j1 = 0; j1 < N1; ++j1)
-          a[j1] = (j1 < ja || j1 > jb) ? row[N2 * j1] : cplx<T>{T(0), T(0)};
+          a[j1] = (j1 < ja || j1 > jb) ? row[S * j1] : cplx<T>{T(0), T(0)};
         sdft<N1, +1>(a);
         twiddle_row<N1, +1>(a, tw, j2);
 #pragma unroll
-        for (int k1 = 0; k1 < N1; ++k1) row[N2 * k1] = a[k1];
+        for (int k1 = 0; k1 < N1; ++k1) row[S * k1] = a[k1];
       }
     }
     wave_sync();
@@ -533,7 +562,7 @@ k_xpass2(GridDev g, cplx<T>* __restrict__ t2, int nslots, int slots_per_wg,
       const int t = lane + 64 * ib;
       if (t < TB) {
         const int r = t / N1, k1 = t % N1;
-        cplx<T>* blk = buf + r * n + N2 * k1;
+        cplx<T>* blk = buf + r * LD + S * k1;
         cplx<T> b[N2];
 #pragma unroll
         for (int j2 = 0; j2 < N2; ++j2) b[j2] = blk[j2];
@@ -577,21 +606,21 @@ k_xpass2(GridDev g, cplx<T>* __restrict__ t2, int nslots, int slots_per_wg,
       const int t = lane + 64 * ia;
       if (t < TA) {
         const int r = t / N2, m2 = t % N2;
-        cplx<T>* row = buf + r * n + m2;
+        cplx<T>* row = buf + r * LD + m2;
         cplx<T> a[N1];
 #pragma unroll
-        for (int k1 = 0; k1 < N1; ++k1) a[k1] = row[N2 * k1];
+        for (int k1 = 0; k1 < N1; ++k1) a[k1] = row[S * k1];
         sdft<N1, -1>(a);
 #pragma unroll
         for (int m1 = 0; m1 < N1; ++m1)
-          if (m1 < ja || m1 > jb) row[N2 * m1] = a[m1];
+          if (m1 < ja || m1 > jb) row[S * m1] = a[m1];
       }
     }
     wave_sync();
     for (int q = lane; q < ng; q += 64) {
       const int dy = q % LPW, lx = q / LPW;
       if (dy < nyl)
-        t2[(((long)s * np2 + z) * nlx + lx) * np1 + y0 + dy] = buf[dy * n + g.line_x[lx]];
+        t2[(((long)s * np2 + z) * nlx + lx) * np1 + y0 + dy] = buf[dy * LD + PD::idx(g.line_x[lx])];
     }
     wave_sync();
   }
@@ -629,15 +658,17 @@ k_zpass_fwd(GridDev g, const cplx<T>* __restrict__ t1, const double* __restrict_
   const int e0 = g.grp_start[grp] + wave * ZLPW;
   const int nl = min(ZLPW, g.grp_start[grp + 1] - e0);
   if (nl <= 0) return;
-  cplx<T>* buf = tw + np2 + wave * ZLPW * np2;
+  using PD = Pad<N1, N2>;
+  const int LD = PD::row(np2);
+  cplx<T>* buf = tw + np2 + wave * ZLPW * LD;
   for (int ll = 0; ll < nl; ++ll) {
     const int col = g.ent_col[e0 + ll];
     const cplx<T>* src = t1 + ((long)slot * g.ncol + (col < 0 ? 0 : col)) * np2;
     for (int z = lane; z < np2; z += 64)
-      buf[ll * np2 + z] = col < 0 ? cplx<T>{T(0), T(0)} : src[z];
+      buf[ll * LD + PD::idx(z)] = col < 0 ? cplx<T>{T(0), T(0)} : src[z];
   }
   wave_sync();
-  line_fft<-1, EPL, N1, N2, ZLPW>(buf, nl, np2, g.ax[2], tw, lane);
+  line_fft<-1, EPL, N1, N2, ZLPW>(buf, nl, LD, g.ax[2], tw, lane);
 
   const double fac = 1.0 / ((double)g.np0 * (double)g.np1 * (double)g.np2);
   int n1, n2;
@@ -659,8 +690,8 @@ k_zpass_fwd(GridDev g, const cplx<T>* __restrict__ t1, const double* __restrict_
   const int first = g.rod_first[rod];
   const int lmin = g.rod_lmin[rod];
   const int size = g.rod_size[rod];
-  const cplx<T>* lp = buf + le * np2;
-  const cplx<T>* lm = (kind == 1) ? buf + (le + 1) * np2 : lp;
+  const cplx<T>* lp = buf + le * LD;
+  const cplx<T>* lm = (kind == 1) ? buf + (le + 1) * LD : lp;
   double* o1 = out + 2 * (long)n1 * ldo;
   double* o2 = out + 2 * (long)(has2 ? n2 : n1) * ldo;
   const double* q1 = psi ? psi + 2 * (long)n1 * ld : nullptr;
@@ -668,10 +699,10 @@ k_zpass_fwd(GridDev g, const cplx<T>* __restrict__ t1, const double* __restrict_
   for (int i = sub; i < size; i += 16) {
     const int l = lmin + i;
     const int ig = first + i;
-    const cplx<T> zp = lp[wrap(l, np2)];
+    const cplx<T> zp = lp[PD::idx(wrap(l, np2))];
     double r1, i1, r2 = 0.0, i2 = 0.0;
     if (MODE == SLOT_PAIR) {
-      const cplx<T> zm = lm[wrap(-l, np2)];
+      const cplx<T> zm = lm[PD::idx(wrap(-l, np2))];
       const double a = double(zp.x) * fac, b = double(zp.y) * fac;
       const double c = double(zm.x) * fac, d = double(zm.y) * fac;
       r1 = 0.5 * (a + c);
@@ -720,17 +751,31 @@ size_t slot_bytes(const Basis* b, int prec) {
   return c * ((size_t)g.ncol * g.np2 + (size_t)g.np2 * g.nlx * g.np1);
 }
 
+// grid edges with a two-factor register transform (must match the dispatch tables)
+static void reg_factors(int n, int& f1, int& f2) {
+  f1 = f2 = 0;
+  switch (n) {
+    case 28: f1 = 7; f2 = 4; break;
+    case 36: f1 = 6; f2 = 6; break;
+    case 72: f1 = 9; f2 = 8; break;
+    case 140: f1 = 14; f2 = 10; break;
+    case 144: f1 = 12; f2 = 12; break;
+    default: break;
+  }
+}
+// LDS row length (Pad<N1,N2>::row) of an edge of length n
+static int lds_row(int n) {
+  int f1, f2;
+  reg_factors(n, f1, f2);
+  return (QBH_LDS_PAD && f1 > 0 && f2 % 2 == 0) ? n + n / f2 : n;
+}
 template <typename T>
 static size_t lds_z(const GridDev& g) {
-  return sizeof(cplx<T>) * (size_t)g.np2 * (1 + NWAVE * ZLPW);
+  return sizeof(cplx<T>) * ((size_t)g.np2 + (size_t)NWAVE * ZLPW * lds_row(g.np2));
 }
 template <typename T>
 static size_t lds_y(const GridDev& g) {
-  return sizeof(cplx<T>) * (size_t)g.np1 * (1 + Cfg<T>::ZB * Cfg<T>::LB);
-}
-template <typename T>
-static size_t lds_x(const GridDev& g) {
-  return sizeof(cplx<T>) * (size_t)g.np0 * (1 + NWAVE * Cfg<T>::LPW);
+  return sizeof(cplx<T>) * ((size_t)g.np1 + (size_t)Cfg<T>::ZB * Cfg<T>::LB * lds_row(g.np1));
 }
 
 static int check_launch(Ctx* ctx, const char* what) {
@@ -760,6 +805,10 @@ static int check_launch(Ctx* ctx, const char* what) {
 #define QBH_SIZE_DISPATCH(n, CALL)                       \
   do {                                                   \
     switch (n) {                                         \
+      case 28: {                                         \
+        constexpr int NM = 144, F1 = 7, F2 = 4;          \
+        CALL;                                            \
+      } break;                                           \
       case 36: {                                         \
         constexpr int NM = 144, F1 = 6, F2 = 6;          \
         CALL;                                            \
@@ -863,13 +912,17 @@ int pipeline_fwd_zy_accumulate(Ctx* ctx, Basis* b, const PipelineArgs& a, double
 
 // slots per x-pass workgroup: keep >= 8192 workgroups in flight while amortising the
 // potential rows and twiddles over several slots
-template <typename T>
-static int x_slots_per_wg(const GridDev& g, int nslots) {
-  constexpr int YB = Cfg<T>::LPW * NWAVE;
-  const long rows = (long)g.np2 * ((g.np1 + YB - 1) / YB);
+static int x_slots_per_wg(const GridDev& g, int nslots, int yb) {
+  const long rows = (long)g.np2 * ((g.np1 + yb - 1) / yb);
   int spw = 1;
   while (spw < 16 && rows * ((nslots + 2 * spw - 1) / (2 * spw)) >= 8192) spw *= 2;
   return spw;
+}
+
+static bool has_x2(int n) {
+  int f1, f2;
+  reg_factors(n, f1, f2);
+  return f1 > 0;
 }
 
 // two-factor register x pass for the grid edges it is instantiated for
@@ -902,15 +955,19 @@ static int xpass_t(Ctx* ctx, Basis* b, const PipelineArgs& a, const double* in, 
                    const double* dens, double scale, int cplx_density, const char* name) {
   const GridDev& g = b->g;
   auto* t2 = static_cast<cplx<T>*>(a.t2);
-  constexpr int YB = Cfg<T>::LPW * NWAVE;
-  const int nyb = (g.np1 + YB - 1) / YB;
-  int spw = (OP == X_DENSITY) ? a.nslots : x_slots_per_wg<T>(g, a.nslots);
+  const bool reg = (OP == X_VMUL || OP == X_DENSITY) && has_x2(g.np0);
+  const int rows = reg ? Cfg<T>::XR : Cfg<T>::LPW;  // rows per wavefront
+  const int yb = rows * NWAVE;
+  const int nyb = (g.np1 + yb - 1) / yb;
+  const size_t lds =
+      sizeof(cplx<T>) * ((size_t)g.np0 + (size_t)NWAVE * rows * (reg ? lds_row(g.np0) : g.np0));
+  int spw = (OP == X_DENSITY) ? a.nslots : x_slots_per_wg(g, a.nslots, yb);
   dim3 grid(g.np2 * nyb, (a.nslots + spw - 1) / spw);
   {
     TimedLaunch tl(ctx, name);
-    if (!launch_x2<T, OP>(ctx, g, grid, lds_x<T>(g), t2, a.nslots, spw, in, out, dens,
-                          cplx_density))
-      QBH_NMAX_DISPATCH(g.np0, (k_xpass<T, OP, NM><<<grid, NTHR, lds_x<T>(g), ctx->stream>>>(
+    if (!(reg && launch_x2<T, OP>(ctx, g, grid, lds, t2, a.nslots, spw, in, out, dens,
+                                  cplx_density)))
+      QBH_NMAX_DISPATCH(g.np0, (k_xpass<T, OP, NM><<<grid, NTHR, lds, ctx->stream>>>(
                                    g, t2, a.nslots, spw, in, out, dens, scale, cplx_density)));
   }
   return check_launch(ctx, name);
