@@ -131,6 +131,13 @@ int qbh_density(qbh_basis basis, const double* psi_dev, long ld, int nst, const 
  * Γ: S = 2 C^T C - c0 c0^T, complex: S = C^H C; S = L L^T; C <- C L^-T (L^-H). */
 int qbh_gram(qbh_basis basis, double* psi_dev, long ld, int nst);
 
+/* Subspace diagonalisation in FP64 (Wavefunction::diag, Wavefunction.cc:1510-1715):
+ * H = 2 C^T (HC) - c0 (hc)0^T (Γ) or C^H (HC); eigenvalues (ascending) into eig (host,
+ * nst entries); if eigvec != 0, C <- C Z in place.  hpsi_dev is left unchanged (the
+ * reference uses dwf as scratch). */
+int qbh_subspace_diag(qbh_basis basis, double* psi_dev, long ld, const double* hpsi_dev,
+                      long ldh, int nst, double* eig, int eigvec);
+
 /* One exponential of the ETRS propagator (ExponentialWavefunctionStepper::exponential,
  * ExponentialWavefunctionStepper.cc:51-148): psi <- sum_{N<=order} (-i dt)^N/N! H^N psi
  * in place, complex basis.  flags as qbh_hpsi (precision bit honoured). */

@@ -212,6 +212,20 @@ def gram(c, real):
     return np.linalg.solve(L.conj(), c.T).T
 
 
+def subspace_diag(c, hc, real):
+    """Wavefunction::diag (Wavefunction.cc:1510-1715) with eigvec=true.  Γ:
+    H = 2 C^T (HC) - c0 (hc)0^T over the real view (gemm 't','n',2.0 then ger -1.0 on
+    row 0, :1546-1548), lower triangle to syevd; complex: H = C^H (HC), heevd.  Returns
+    (eigenvalues ascending, C Z)."""
+    if real:
+        h = 2.0 * (c.real.T @ hc.real + c.imag.T @ hc.imag) - np.outer(c[0].real, hc[0].real)
+        w, z = np.linalg.eigh(h, UPLO="L")
+    else:
+        h = c.conj().T @ hc
+        w, z = np.linalg.eigh(h, UPLO="L")
+    return w, c @ z
+
+
 # --------------------------------------------------------------------------------------
 def exponential(apply_h, c, dt, order=4):
     """ExponentialWavefunctionStepper::exponential, one exponential (num_exp=1)

@@ -50,6 +50,7 @@ SIGNATURES = {
     "qbh_hpsi": (_c_int, [_vp, _vp, _c_long, _c_int, _dp, _vp, _c_long, _c_int, _dp]),
     "qbh_density": (_c_int, [_vp, _vp, _c_long, _c_int, _dp, _c_double, _c_double, _vp, _c_int]),
     "qbh_gram": (_c_int, [_vp, _vp, _c_long, _c_int]),
+    "qbh_subspace_diag": (_c_int, [_vp, _vp, _c_long, _vp, _c_long, _c_int, _dp, _c_int]),
     "qbh_exponential": (_c_int, [_vp, _vp, _c_long, _c_int, _c_double, _c_int, _c_int]),
 }
 
@@ -187,6 +188,13 @@ class Basis:
 
     def gram(self, psi_ptr, ld, nst):
         self.ctx.check(lib().qbh_gram(self.h, psi_ptr, ld, nst), "qbh_gram")
+
+    def subspace_diag(self, psi_ptr, ld, hpsi_ptr, ldh, nst, eigvec=True):
+        w = np.zeros(nst)
+        self.ctx.check(lib().qbh_subspace_diag(self.h, psi_ptr, ld, hpsi_ptr, ldh, nst,
+                                               w.ctypes.data_as(_dp), int(eigvec)),
+                       "qbh_subspace_diag")
+        return w
 
     def close(self):
         if self.h:

@@ -443,63 +443,6 @@ extern "C" int qbh_density(qbh_basis basis, const double* psi_dev, long ld, int 
 }
 
 // ------------------------------------------------------------------------------------
-extern "C" int qbh_gram(qbh_basis basis, double* psi_dev, long ld, int nst) {
-  if (!basis || !psi_dev || nst <= 0) return QBH_ERR_ARG;
-  Basis* b = basis_of(basis);
-  Ctx* c = ctx_of_basis(basis);
-  hipSetDevice(c->device);
-  const int ngw = b->g.ngw;
-  if (ld < ngw) return fail(c, QBH_ERR_ARG, "ld < ngw");
-  void* hb;
-  if (int r = blas_handle(c, &hb)) return r;
-  rocblas_handle bh = static_cast<rocblas_handle>(hb);
-  const bool real = b->g.real;
-  double* S = nullptr;
-  rocblas_int* info = nullptr;
-  const size_t ssz = (size_t)nst * nst * (real ? 1 : 2);
-  if (hipMalloc(&S, ssz * sizeof(double)) != hipSuccess ||
-      hipMalloc(&info, sizeof(rocblas_int)) != hipSuccess) {
-    if (S) hipFree(S);
-    return fail(c, QBH_ERR_ALLOC, "gram scratch allocation failed");
-  }
-  rocblas_status st;
-  if (real) {
-    const double two = 2.0, zero = 0.0, mone = -1.0, one = 1.0;
-    // S = 2 C^T C - c0 c0^T (SlaterDet.cc:1065-1066), lower triangle
-    st = rocblas_dsyrk(bh, rocblas_fill_lower, rocblas_operation_transpose, nst, 2 * ngw, &two,
-                       psi_dev, 2 * ld, &zero, S, nst);
-    if (st == rocblas_status_success)
-      st = rocblas_dsyr(bh, rocblas_fill_lower, nst, &mone, psi_dev, 2 * ld, S, nst);
-    if (st == rocblas_status_success)
-      st = rocsolver_dpotrf(bh, rocblas_fill_lower, nst, S, nst, info);
-    if (st == rocblas_status_success)
-      st = rocblas_dtrsm(bh, rocblas_side_right, rocblas_fill_lower, rocblas_operation_transpose,
-                         rocblas_diagonal_non_unit, 2 * ngw, nst, &one, S, nst, psi_dev, 2 * ld);
-  } else {
-    const double one = 1.0, zero = 0.0;
-    const rocblas_double_complex zone(1.0, 0.0);
-    auto* C = reinterpret_cast<rocblas_double_complex*>(psi_dev);
-    auto* Sz = reinterpret_cast<rocblas_double_complex*>(S);
-    st = rocblas_zherk(bh, rocblas_fill_lower, rocblas_operation_conjugate_transpose, nst, ngw,
-                       &one, C, ld, &zero, Sz, nst);
-    if (st == rocblas_status_success)
-      st = rocsolver_zpotrf(bh, rocblas_fill_lower, nst, Sz, nst, info);
-    if (st == rocblas_status_success)
-      st = rocblas_ztrsm(bh, rocblas_side_right, rocblas_fill_lower,
-                         rocblas_operation_conjugate_transpose, rocblas_diagonal_non_unit, ngw,
-                         nst, &zone, Sz, nst, C, ld);
-  }
-  rocblas_int hinfo = 0;
-  hipMemcpyAsync(&hinfo, info, sizeof(rocblas_int), hipMemcpyDeviceToHost, c->stream);
-  hipStreamSynchronize(c->stream);
-  hipFree(S);
-  hipFree(info);
-  if (st != rocblas_status_success)
-    return fail(c, QBH_ERR_BLAS, std::string("gram: ") + rocblas_status_to_string(st));
-  if (hinfo != 0) return fail(c, QBH_ERR_ARG, "gram: overlap matrix not positive definite");
-  return dev_check(c, "gram");
-}
-
 // ------------------------------------------------------------------------------------
 __global__ void k_zaxpy(long n, double ar, double ai, const double* __restrict__ x, long ldx,
                         double* __restrict__ y, long ldy, int ngw) {

@@ -299,3 +299,39 @@ def test_noncubic_grids_generic_and_register_paths(gpu_ctx, si8, grid):
     assert rel_err(rho.cpu().numpy(), oh.compute_density(maps, si8.psi, si8.occ, 1.0, si8.omega)) <= FP64_TOL
     ham.close()
     basis.close()
+
+
+@pytest.mark.parametrize("which", ["si8", "si8c"])
+def test_subspace_diag(gpu_ctx, which, request):
+    import torch
+    from oracle import hpsi as oh
+    sysm = request.getfixturevalue(which)
+    ob, maps, sp = oracle_for(sysm)
+    c = oh.gram(sysm.psi, sysm.real)
+    hc, _ = oh.hpsi(maps, c, sysm.v, [sp], sysm.occ, sysm.omega)
+    w_ref, cz_ref = oh.subspace_diag(c, hc, sysm.real)
+    basis = make_gpu_basis(gpu_ctx, sysm)
+    cd = to_dev(c)
+    hd = to_dev(hc)
+    w = basis.subspace_diag(cd.data_ptr(), basis.ngw, hd.data_ptr(), basis.ngw, sysm.nst)
+    gpu_ctx.sync()
+    cz = from_dev(cd, basis.ngw, sysm.nst)
+    basis.close()
+    assert np.max(np.abs(w - w_ref)) <= 1e-12 * np.max(np.abs(w_ref))
+    # eigenvectors are fixed up to a phase per column (sign for Γ)
+    for n in range(sysm.nst):
+        ph = np.vdot(cz[:, n], cz_ref[:, n])
+        ph /= abs(ph)
+        assert rel_err(cz[:, n] * ph, cz_ref[:, n]) <= 1e-10
+
+
+def test_gram_si64_split_k(gpu_ctx, si64):
+    """Si64 (18278-row real view): the overlap product runs K-split."""
+    from oracle import hpsi as oh
+    basis = make_gpu_basis(gpu_ctx, si64)
+    cd = to_dev(si64.psi)
+    basis.gram(cd.data_ptr(), basis.ngw, si64.nst)
+    gpu_ctx.sync()
+    got = from_dev(cd, basis.ngw, si64.nst)
+    basis.close()
+    assert rel_err(got, oh.gram(si64.psi, True)) <= 1e-11

@@ -170,3 +170,21 @@ def test_kpgpa2():
     A = np.array([0.1, -0.2, 0.05])
     d = b.kpgx - A[None, :]
     assert np.max(np.abs(oh.kpgpa2(b, A) - np.sum(d * d, axis=1))) <= 1e-12
+
+
+@pytest.mark.parametrize("real", [True, False])
+def test_subspace_diag_identities(real):
+    """H Z = Z w in the Γ metric: <Cz_i|H|Cz_j> = w_i delta_ij for orthonormal C, and the
+    rotated block stays orthonormal; H built independently over the full sphere."""
+    cell, b, maps, c, rng = setup(real, 5)
+    c = oh.gram(c, real)
+    sp = species(b, rng)
+    v = rng.standard_normal(maps.N)
+    hc, _ = oh.hpsi(maps, c, v, [sp], np.zeros(5), cell.volume)
+    w, cz = oh.subspace_diag(c, hc, real)
+    assert np.all(np.diff(w) >= 0)
+    hcz, _ = oh.hpsi(maps, cz, v, [sp], np.zeros(5), cell.volume)
+    m = np.array([[gdot(cz[:, [i]], hcz[:, [j]], real)[0] for j in range(5)] for i in range(5)])
+    assert np.max(np.abs(m - np.diag(w))) <= 1e-10 * np.max(np.abs(w))
+    s = np.array([[gdot(cz[:, [i]], cz[:, [j]], real)[0] for j in range(5)] for i in range(5)])
+    assert np.max(np.abs(s - np.eye(5))) <= 1e-12
