@@ -138,6 +138,19 @@ int qbh_gram(qbh_basis basis, double* psi_dev, long ld, int nst);
 int qbh_subspace_diag(qbh_basis basis, double* psi_dev, long ld, const double* hpsi_dev,
                       long ldh, int nst, double* eig, int eigvec);
 
+/* Self-consistent local potential on the density grid (EnergyFunctional::update_vhxc,
+ * EnergyFunctional.cc:353-977; LDA PZ81 of LDAFunctional::xc_unpolarized, nspin = 1):
+ *   v_r = v_xc(rho) + FFT^-1[v_ion(G) + 4 pi (rhog/omega + rho_ps(G)) / G^2]
+ * vbasis: the density basis (ChargeDensity::vbasis_) on the same grid as rho.  rhog_dev is
+ * ChargeDensity::rhog (forward transform of omega*rho, as qbh_fft_forward_real gives);
+ * hamil_rhog_dev (nullable) replaces rhog in the Hartree term (TDDFT Hamiltonian density,
+ * EnergyFunctional.cc:473-487); vion_g_dev, rhops_g_dev (nullable) are the ionic local
+ * potential and pseudo-charge on the density basis.  energies (nullable, host) receives
+ * {eps, ehart, exc}. */
+int qbh_update_vhxc(qbh_basis vbasis, const double* rhor_dev, const double* rhog_dev,
+                    const double* hamil_rhog_dev, const double* vion_g_dev,
+                    const double* rhops_g_dev, double omega, double* v_r_dev, double* energies);
+
 /* One exponential of the ETRS propagator (ExponentialWavefunctionStepper::exponential,
  * ExponentialWavefunctionStepper.cc:51-148): psi <- sum_{N<=order} (-i dt)^N/N! H^N psi
  * in place, complex basis.  flags as qbh_hpsi (precision bit honoured). */

@@ -51,6 +51,7 @@ SIGNATURES = {
     "qbh_density": (_c_int, [_vp, _vp, _c_long, _c_int, _dp, _c_double, _c_double, _vp, _c_int]),
     "qbh_gram": (_c_int, [_vp, _vp, _c_long, _c_int]),
     "qbh_subspace_diag": (_c_int, [_vp, _vp, _c_long, _vp, _c_long, _c_int, _dp, _c_int]),
+    "qbh_update_vhxc": (_c_int, [_vp, _vp, _vp, _vp, _vp, _vp, _c_double, _vp, _dp]),
     "qbh_exponential": (_c_int, [_vp, _vp, _c_long, _c_int, _c_double, _c_int, _c_int]),
 }
 
@@ -185,6 +186,15 @@ class Basis:
         o_, op = _da(occ)
         self.ctx.check(lib().qbh_density(self.h, psi_ptr, ld, nst, op, float(weight), float(omega),
                                          rho_ptr, flags), "qbh_density")
+
+    def update_vhxc(self, rhor_ptr, rhog_ptr, vion_g_ptr, rhops_g_ptr, omega, v_r_ptr,
+                    hamil_rhog_ptr=None):
+        """EnergyFunctional::update_vhxc on this (density) basis; returns (eps, ehart, exc)."""
+        e = np.zeros(3)
+        self.ctx.check(lib().qbh_update_vhxc(self.h, rhor_ptr, rhog_ptr, hamil_rhog_ptr, vion_g_ptr,
+                                             rhops_g_ptr, float(omega), v_r_ptr,
+                                             e.ctypes.data_as(_dp)), "qbh_update_vhxc")
+        return tuple(e)
 
     def gram(self, psi_ptr, ld, nst):
         self.ctx.check(lib().qbh_gram(self.h, psi_ptr, ld, nst), "qbh_gram")

@@ -335,3 +335,38 @@ def test_gram_si64_split_k(gpu_ctx, si64):
     got = from_dev(cd, basis.ngw, si64.nst)
     basis.close()
     assert rel_err(got, oh.gram(si64.psi, True)) <= 1e-11
+
+
+@pytest.mark.parametrize("which,tddft", [("si8", False), ("si8c", False), ("si8c", True)])
+def test_update_vhxc(gpu_ctx, which, tddft, request):
+    import torch
+    from oracle import hpsi as oh
+    from oracle.basis import Basis as OB, FFTMaps, UnitCell
+    from oracle.vhxc import update_vhxc
+    from qball_amd import Basis, capi
+    sysm = request.getfixturevalue(which)
+    ob, maps, _ = oracle_for(sysm)
+    vob = OB(UnitCell(sysm.a), 4.0 * sysm.ecut, force_complex=not sysm.real)
+    vmaps = FFTMaps(vob, *sysm.np)
+    c = oh.gram(sysm.psi, sysm.real)
+    rho, nel, rhog = oh.update_density(maps, vmaps, c, sysm.occ, 1.0, sysm.omega)
+    rng = np.random.default_rng(21)
+    vion = (rng.standard_normal(vob.size) + 1j * rng.standard_normal(vob.size)) * np.exp(-0.3 * vob.g2)
+    rhops = (0.05 * np.exp(-0.5 * vob.g2)).astype(np.complex128)
+    if sysm.real:
+        vion[0] = vion[0].real
+    hrhog = rhog * (1.0 + 0.01 * rng.standard_normal(vob.size)) if tddft else None
+    v_ref, eps, eh, exc = update_vhxc(vmaps, rho, rhog, vion, rhops, sysm.omega, hamil_rhog=hrhog)
+    r = capi.rods_build(sysm.a, 4.0 * sysm.ecut, sysm.real)
+    vb = Basis(gpu_ctx, sysm.real, r["rod_h"], r["rod_k"], r["rod_lmin"], r["rod_size"], sysm.np,
+               sysm.b)
+    cplx = lambda z: torch.from_numpy(np.ascontiguousarray(z).view(np.float64).copy()).cuda()
+    rd, gd, vd, pd = real_to_dev(rho), cplx(rhog), cplx(vion), cplx(rhops)
+    hd = cplx(hrhog) if tddft else None
+    out = torch.zeros(vb.N, dtype=torch.float64, device="cuda")
+    e = vb.update_vhxc(rd.data_ptr(), gd.data_ptr(), vd.data_ptr(), pd.data_ptr(), sysm.omega,
+                       out.data_ptr(), hamil_rhog_ptr=hd.data_ptr() if tddft else None)
+    gpu_ctx.sync()
+    vb.close()
+    assert rel_err(out.cpu().numpy(), v_ref) <= FP64_TOL
+    assert abs(e[0] - eps) <= 1e-10 and abs(e[1] - eh) <= 1e-10 and abs(e[2] - exc) <= 1e-10

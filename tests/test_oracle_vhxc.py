@@ -1,0 +1,74 @@
+"""Oracle update_vhxc pinned by closed forms and real-space identities."""
+import math
+
+import numpy as np
+import pytest
+
+from oracle import fft as offt
+from oracle import hpsi as oh
+from oracle.basis import Basis, FFTMaps, UnitCell
+from oracle.vhxc import pz81, update_vhxc
+
+
+def test_pz81_exchange_closed_form_and_continuity():
+    rho = np.logspace(-6, 2, 200)
+    ee, vv = pz81(rho)
+    # Slater exchange: e_x = -(3/4) (3 rho / pi)^(1/3), v_x = (4/3) e_x
+    ex = -0.75 * np.cbrt(3.0 * rho / math.pi)
+    rs = np.cbrt(3.0 / (4.0 * math.pi * rho))
+    # correlation alone, high- and low-density branches agree at rs = 1 (value and potential)
+    A, B, b1, b2, G = 0.0311, -0.048, 1.0529, 0.3334, -0.1423
+    ec_hi = G / (1 + b1 + b2)
+    ec_lo = B + (G / (1 + b1 + b2) - B)  # A log 1 = 0, C*0 ... + D
+    assert abs(ec_hi - ec_lo) < 1e-15
+    # the function is continuous across rs = 1
+    rho1 = 3.0 / (4.0 * math.pi)
+    e_a, v_a = pz81(np.array([rho1 * (1 + 1e-9), rho1 * (1 - 1e-9)]))
+    assert abs(e_a[0] - e_a[1]) < 1e-9 and abs(v_a[0] - v_a[1]) < 1e-9
+    # exchange part: subtract correlation evaluated from the published formula
+    ec = np.where(rs < 1.0, 0.0, G / (1 + b1 * np.sqrt(rs) + b2 * rs))
+    m = rs >= 1.0
+    assert np.max(np.abs((ee - ec)[m] - ex[m]) / np.abs(ex[m])) < 1e-12
+    # v_xc = d(rho e_xc)/d rho (central differences)
+    h = 1e-6
+    d = ((rho * (1 + h)) * pz81(rho * (1 + h))[0] - (rho * (1 - h)) * pz81(rho * (1 - h))[0]) / (2 * h * rho)
+    assert np.max(np.abs(d - vv) / np.abs(vv)) < 1e-7
+    assert pz81(np.array([0.0, -1.0]))[0].tolist() == [0.0, 0.0]
+
+
+@pytest.mark.parametrize("real", [True, False])
+def test_vhxc_energies_by_real_space_routes(real):
+    cell = UnitCell(np.array([[6.0, 0.2, 0.0], [0.0, 5.5, 0.4], [0.3, 0.0, 6.5]]))
+    wb = Basis(cell, 3.0, force_complex=not real)
+    vb = Basis(cell, 12.0, force_complex=not real)
+    n = [max(a, b) for a, b in zip(wb.np, vb.np)]
+    maps = FFTMaps(wb, *n)
+    vmaps = FFTMaps(vb, *n)
+    rng = np.random.default_rng(4)
+    c = rng.standard_normal((wb.size, 3)) + 1j * rng.standard_normal((wb.size, 3))
+    if real:
+        c[0] = c[0].real
+    c = oh.gram(c, real)
+    occ = np.array([2.0, 2.0, 1.0])
+    rho, nel, rhog = oh.update_density(maps, vmaps, c, occ, 1.0, cell.volume)
+    vion = rng.standard_normal(vb.size) + 1j * rng.standard_normal(vb.size)
+    vion *= np.exp(-0.3 * vb.g2)
+    rhops = 0.1 * np.exp(-0.5 * vb.g2).astype(np.complex128)
+    if real:
+        vion[0], rhops[0] = vion[0].real, rhops[0].real
+    vr, eps, ehart, exc = update_vhxc(vmaps, rho, rhog, vion, rhops, cell.volume)
+    N = vmaps.N
+    dv = cell.volume / N
+    # eps = integral rho(r) v_ion(r)
+    vion_r = offt.backward(vmaps, vion).ravel()
+    assert abs(eps - dv * np.sum(rho * vion_r.real)) <= 1e-10 * max(1.0, abs(eps))
+    # ehart = 1/2 integral rho_t v_H with v_H = FFT^-1[4 pi rho_t(G)/G^2]
+    g = np.sqrt(vb.g2)
+    g2i = np.where(g > 0, 1.0 / np.where(g > 0, g * g, 1.0), 0.0)
+    rhot = rhog / cell.volume + rhops
+    vh_r = offt.backward(vmaps, 4 * math.pi * rhot * g2i).ravel().real
+    rhot_r = offt.backward(vmaps, rhot).ravel().real
+    assert abs(ehart - 0.5 * dv * np.sum(rhot_r * vh_r)) <= 1e-10 * abs(ehart)
+    # v_r = v_xc + v_ion + v_H on the grid
+    assert np.max(np.abs(vr - (pz81(rho)[1] + vion_r.real + vh_r))) <= 1e-12 * np.max(np.abs(vr))
+    assert abs(exc - dv * np.sum(rho * pz81(rho)[0])) <= 1e-13 * abs(exc)
