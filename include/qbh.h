@@ -157,6 +157,24 @@ int qbh_update_vhxc(qbh_basis vbasis, const double* rhor_dev, const double* rhog
 int qbh_exponential(qbh_ham ham, double* psi_dev, long ld, int nst, double dt, int order,
                     int flags);
 
+/* One ETRS step with a self-consistent H(t + dt) (ExponentialWavefunctionStepper::preupdate
+ * and update, ExponentialWavefunctionStepper.cc:151-276, merge_exp_ false, ETRS not AETRS),
+ * complex basis:
+ *   half  = T(dt/2) psi under H(t);  tilde = T(dt/2) half under H(t)   (preupdate)
+ *   rho(tilde) and rhog = forward(omega rho)            (hamil_cd_->update_density())
+ *   v_r of ham <- update_vhxc(vbasis, rho, rhog, hamil_rhog = rhog, vion_g, rhops_g)
+ *                                     (update_hamiltonian(); update_vhxc(), :259-261)
+ *   psi   = T(dt/2) half under H(t + dt)                                (update)
+ * T = the order-`order` Taylor exponential of qbh_exponential.  vbasis must share the
+ * wavefunction grid (ChargeDensity builds its wavefunction transforms on the density
+ * grid, ChargeDensity.cc:145).  occ (host) and weight feed the density; energies
+ * (nullable, host) receives {eps, ehart, exc} of the H(t + dt) density.  On return ham
+ * holds H(t + dt)'s v_r. */
+int qbh_etrs_step(qbh_ham ham, qbh_basis vbasis, double* psi_dev, long ld, int nst,
+                  const double* occ, double weight, double dt, int order,
+                  const double* vion_g_dev, const double* rhops_g_dev, double omega,
+                  double* energies, int flags);
+
 #ifdef __cplusplus
 }
 #endif

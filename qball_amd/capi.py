@@ -53,6 +53,8 @@ SIGNATURES = {
     "qbh_subspace_diag": (_c_int, [_vp, _vp, _c_long, _vp, _c_long, _c_int, _dp, _c_int]),
     "qbh_update_vhxc": (_c_int, [_vp, _vp, _vp, _vp, _vp, _vp, _c_double, _vp, _dp]),
     "qbh_exponential": (_c_int, [_vp, _vp, _c_long, _c_int, _c_double, _c_int, _c_int]),
+    "qbh_etrs_step": (_c_int, [_vp, _vp, _vp, _c_long, _c_int, _dp, _c_double, _c_double, _c_int,
+                               _vp, _vp, _c_double, _dp, _c_int]),
 }
 
 _lib = None
@@ -241,6 +243,18 @@ class Hamiltonian:
     def exponential(self, psi_ptr, ld, nst, dt, order=4, flags=QBH_ALL):
         self.ctx.check(lib().qbh_exponential(self.h, psi_ptr, ld, nst, float(dt), int(order),
                                              flags), "qbh_exponential")
+
+    def etrs_step(self, vbasis, psi_ptr, ld, nst, occ, weight, dt, vion_g_ptr, rhops_g_ptr,
+                  omega, order=4, flags=QBH_ALL):
+        """One self-consistent ETRS step (preupdate + update); returns the {eps, ehart, exc}
+        of the H(t+dt) density.  v_r of this Hamiltonian becomes H(t+dt)'s."""
+        o = np.ascontiguousarray(occ, dtype=np.float64)
+        e = np.zeros(3)
+        self.ctx.check(lib().qbh_etrs_step(self.h, vbasis.h, psi_ptr, ld, nst, o.ctypes.data_as(_dp),
+                                           float(weight), float(dt), int(order), vion_g_ptr,
+                                           rhops_g_ptr, float(omega), e.ctypes.data_as(_dp),
+                                           flags), "qbh_etrs_step")
+        return tuple(e)
 
     def close(self):
         if self.h:

@@ -186,6 +186,7 @@ extern "C" int qbh_ham_create(qbh_ctx ctx, qbh_basis basis, qbh_ham* out) {
   auto* hh = new qbh_ham_s();
   hh->h.ctx = c;
   hh->h.b = basis_of(basis);
+  hh->h.b_handle = basis;
   if (int r = upload_kin(&hh->h)) {
     delete hh;
     return r;
@@ -211,7 +212,8 @@ extern "C" int qbh_ham_destroy(qbh_ham ham) {
   hipSetDevice(h->ctx->device);
   hipStreamSynchronize(h->ctx->stream);
   free_proj(h);
-  for (void* p : {(void*)h->v, (void*)h->kin, h->f64, h->f32, h->psi32, h->tmp32, (void*)h->red})
+  for (void* p : {(void*)h->v, (void*)h->kin, h->f64, h->f32, h->psi32, h->tmp32, (void*)h->red,
+                  h->etrs})
     if (p) hipFree(p);
   for (void* p : h->expo) hipFree(p);
   delete ham;
@@ -456,18 +458,17 @@ __global__ void k_zaxpy(long n, double ar, double ai, const double* __restrict__
   yp[1] += ar * xi + ai * xr;
 }
 
-extern "C" int qbh_exponential(qbh_ham ham, double* psi_dev, long ld, int nst, double dt,
-                               int order, int flags) {
-  if (!ham || !psi_dev || nst <= 0 || order < 1) return QBH_ERR_ARG;
-  Ham* h = &ham->h;
+// out = sum_{N<=order} (-i dt)^N/N! H^N in (ExponentialWavefunctionStepper::exponential,
+// ExponentialWavefunctionStepper.cc:51-148, num_exp = 1); out may alias in.
+static int expo_impl(Ham* h, const double* in, long ldin, double* outp, long ldout, int nst,
+                     double dt, int order, int flags) {
   Ctx* c = h->ctx;
-  hipSetDevice(c->device);
   const int ngw = h->b->g.ngw;
-  if (ld < ngw) return fail(c, QBH_ERR_ARG, "ld < ngw");
   const size_t bytes = 2 * sizeof(double) * (size_t)ngw * nst;
   if (h->expo.size() != 3 || h->expo_bytes < bytes) {
     for (void* p : h->expo) hipFree(p);
     h->expo.assign(3, nullptr);
+    h->expo_bytes = 0;
     for (auto& p : h->expo)
       if (hipMalloc(&p, bytes) != hipSuccess) {
         p = nullptr;
@@ -477,13 +478,13 @@ extern "C" int qbh_exponential(qbh_ham ham, double* psi_dev, long ld, int nst, d
   }
   double* expwf = static_cast<double*>(h->expo[0]);
   double* bufs[2] = {static_cast<double*>(h->expo[1]), static_cast<double*>(h->expo[2])};
-  // expwf = psi (compact, ld = ngw)
-  hipMemcpy2DAsync(expwf, 2 * sizeof(double) * ngw, psi_dev, 2 * sizeof(double) * ld,
+  // expwf = in (compact, ld = ngw)
+  hipMemcpy2DAsync(expwf, 2 * sizeof(double) * ngw, in, 2 * sizeof(double) * ldin,
                    2 * sizeof(double) * ngw, nst, hipMemcpyDeviceToDevice, c->stream);
   std::vector<double> occ(nst, 0.0);
   double fr = 1.0, fi = 0.0;  // factor
-  const double* in = psi_dev;
-  long ldin = ld;
+  const double* x = in;
+  long ldx = ldin;
   const int hflags = (flags & (QBH_ALL | QBH_FP32)) == 0 ? QBH_ALL : (flags & (QBH_ALL | QBH_FP32));
   for (int N = 1; N <= order; ++N) {
     // factor *= -i*dt/N (ExponentialWavefunctionStepper.cc:97-100)
@@ -492,14 +493,72 @@ extern "C" int qbh_exponential(qbh_ham ham, double* psi_dev, long ld, int nst, d
     fr = nr;
     fi = ni;
     double* out = bufs[(N - 1) & 1];
-    if (int r = hpsi_impl(h, in, ldin, nst, occ.data(), out, ngw, hflags, nullptr)) return r;
+    if (int r = hpsi_impl(h, x, ldx, nst, occ.data(), out, ngw, hflags, nullptr)) return r;
     const long n = (long)ngw * nst;
     k_zaxpy<<<(unsigned)((n + 255) / 256), 256, 0, c->stream>>>(n, fr, fi, out, ngw, expwf, ngw,
                                                                 ngw);
-    in = out;
-    ldin = ngw;
+    x = out;
+    ldx = ngw;
   }
-  hipMemcpy2DAsync(psi_dev, 2 * sizeof(double) * ld, expwf, 2 * sizeof(double) * ngw,
+  hipMemcpy2DAsync(outp, 2 * sizeof(double) * ldout, expwf, 2 * sizeof(double) * ngw,
                    2 * sizeof(double) * ngw, nst, hipMemcpyDeviceToDevice, c->stream);
   return dev_check(c, "exponential");
+}
+
+extern "C" int qbh_exponential(qbh_ham ham, double* psi_dev, long ld, int nst, double dt,
+                               int order, int flags) {
+  if (!ham || !psi_dev || nst <= 0 || order < 1) return QBH_ERR_ARG;
+  Ham* h = &ham->h;
+  Ctx* c = h->ctx;
+  hipSetDevice(c->device);
+  if (ld < h->b->g.ngw) return fail(c, QBH_ERR_ARG, "ld < ngw");
+  if (h->b->g.real) return fail(c, QBH_ERR_ARG, "exponential needs a complex basis");
+  return expo_impl(h, psi_dev, ld, psi_dev, ld, nst, dt, order, flags);
+}
+
+// One ETRS step (ExponentialWavefunctionStepper::preupdate + update,
+// ExponentialWavefunctionStepper.cc:151-276, merge_exp_ false, not AETRS):
+//   half  = T(dt/2, H(t)) psi            psi(t + dt/2)
+//   tilde = T(dt/2, H(t)) half           approximation to psi(t + dt) (hamil_wf)
+//   rho, rhog <- density(tilde)          hamil_cd_->update_density()
+//   v_r <- update_vhxc(rho, rhog)        update_hamiltonian(); update_vhxc()
+//   psi   = T(dt/2, H(t + dt)) half
+extern "C" int qbh_etrs_step(qbh_ham ham, qbh_basis vbasis, double* psi_dev, long ld, int nst,
+                             const double* occ, double weight, double dt, int order,
+                             const double* vion_g_dev, const double* rhops_g_dev, double omega,
+                             double* energies, int flags) {
+  if (!ham || !vbasis || !psi_dev || !occ || nst <= 0 || order < 1 || !(omega > 0.0))
+    return QBH_ERR_ARG;
+  Ham* h = &ham->h;
+  Ctx* c = h->ctx;
+  hipSetDevice(c->device);
+  const GridDev& g = h->b->g;
+  const GridDev& vg = basis_of(vbasis)->g;
+  if (ld < g.ngw) return fail(c, QBH_ERR_ARG, "ld < ngw");
+  if (g.real) return fail(c, QBH_ERR_ARG, "etrs_step needs a complex basis");
+  if (ctx_of_basis(vbasis) != c) return fail(c, QBH_ERR_ARG, "vbasis on another context");
+  if (vg.np0 != g.np0 || vg.np1 != g.np1 || vg.np2 != g.np2)
+    return fail(c, QBH_ERR_ARG, "density basis grid differs from the wavefunction grid");
+  if (!h->v) return fail(c, QBH_ERR_ARG, "v_r not set");
+  const long N = (long)g.np0 * g.np1 * g.np2;
+  const size_t wbytes = 2 * sizeof(double) * (size_t)g.ngw * nst;
+  const size_t need = 2 * wbytes + sizeof(double) * (size_t)(N + 2L * vg.ngw);
+  if (!grow(h->etrs, h->etrs_bytes, need))
+    return fail(c, QBH_ERR_ALLOC, "etrs buffers allocation failed");
+  double* half = static_cast<double*>(h->etrs);
+  double* tilde = half + 2 * (size_t)g.ngw * nst;
+  double* rho = tilde + 2 * (size_t)g.ngw * nst;
+  double* rhog = rho + N;
+  const int fl = flags & (QBH_ALL | QBH_FP32);
+  if (int r = expo_impl(h, psi_dev, ld, half, g.ngw, nst, 0.5 * dt, order, fl)) return r;
+  if (int r = expo_impl(h, half, g.ngw, tilde, g.ngw, nst, 0.5 * dt, order, fl)) return r;
+  fill0(c, rho, N);
+  if (int r = qbh_density(reinterpret_cast<qbh_basis>(h->b_handle), tilde, g.ngw, nst, occ,
+                          weight, omega, rho, flags & QBH_FP32))
+    return r;
+  if (int r = qbh_fft_forward_real(vbasis, rho, omega, rhog, 0)) return r;
+  if (int r = qbh_update_vhxc(vbasis, rho, rhog, rhog, vion_g_dev, rhops_g_dev, omega, h->v,
+                              energies))
+    return r;
+  return expo_impl(h, half, g.ngw, psi_dev, ld, nst, 0.5 * dt, order, fl);
 }

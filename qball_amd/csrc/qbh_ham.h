@@ -9,6 +9,7 @@ namespace qbh {
 struct Ham {
   Ctx* ctx = nullptr;
   Basis* b = nullptr;
+  void* b_handle = nullptr;  // the qbh_basis handle b came from
   double* v = nullptr;    // v_r on the grid (owned copy)
   double* kin = nullptr;  // 0.5 |k+G-A|^2 per plane wave
   double A[3] = {0, 0, 0};
@@ -40,6 +41,8 @@ struct Ham {
   std::vector<double> occ_host;   // occupations currently on the device
   std::vector<void*> expo;        // exponential buffers
   size_t expo_bytes = 0;
+  void* etrs = nullptr;           // ETRS step: psi(t+dt/2), psi~(t+dt), rho, rhog
+  size_t etrs_bytes = 0;
 };
 
 int nl_apply(Ham* h, const double* psi, long ld, int nst, const double* occ, double* hpsi,

@@ -370,3 +370,57 @@ def test_update_vhxc(gpu_ctx, which, tddft, request):
     vb.close()
     assert rel_err(out.cpu().numpy(), v_ref) <= FP64_TOL
     assert abs(e[0] - eps) <= 1e-10 and abs(e[1] - eh) <= 1e-10 and abs(e[2] - exc) <= 1e-10
+
+
+def test_etrs_step_selfconsistent_si8c(gpu_ctx, si8c):
+    """qbh_etrs_step against the oracle's ETRS step with H(t+dt) rebuilt from the density
+    of psi~(t+dt) (update_density + update_vhxc), two steps in a row."""
+    import torch
+    from oracle import hpsi as oh
+    from oracle.basis import Basis as OB, FFTMaps, UnitCell
+    from oracle.vhxc import update_vhxc
+    from qball_amd import Basis, capi
+    sysm = si8c
+    ob, maps, sp = oracle_for(sysm)
+    vob = OB(UnitCell(sysm.a), 4.0 * sysm.ecut, force_complex=True)
+    vmaps = FFTMaps(vob, *sysm.np)
+    rng = np.random.default_rng(5)
+    vion = (rng.standard_normal(vob.size) + 1j * rng.standard_normal(vob.size)) * 0.5 * np.exp(-0.3 * vob.g2)
+    rhops = (0.05 * np.exp(-0.5 * vob.g2)).astype(np.complex128)
+    occ0 = np.zeros(sysm.nst)
+    psi = oh.gram(sysm.psi, False)
+    dt = 0.05
+
+    def oracle_step(c, v):
+        def apply_t(w):
+            return oh.hpsi(maps, w, v, [sp], occ0, sysm.omega)[0]
+
+        state = {}
+
+        def h_tdt(tilde):
+            rho, _, rhog = oh.update_density(maps, vmaps, tilde, sysm.occ, 1.0, sysm.omega)
+            v2, eps, eh, exc = update_vhxc(vmaps, rho, rhog, vion, rhops, sysm.omega, hamil_rhog=rhog)
+            state.update(v=v2, e=(eps, eh, exc))
+            return lambda w: oh.hpsi(maps, w, v2, [sp], occ0, sysm.omega)[0]
+        out, _ = oh.etrs_step(apply_t, h_tdt, c, dt)
+        return out, state["v"], state["e"]
+
+    r = capi.rods_build(sysm.a, 4.0 * sysm.ecut, False)
+    vb = Basis(gpu_ctx, False, r["rod_h"], r["rod_k"], r["rod_lmin"], r["rod_size"], sysm.np, sysm.b)
+    basis = make_gpu_basis(gpu_ctx, sysm)
+    ham = make_gpu_ham(gpu_ctx, basis, sysm)
+    cplx = lambda z: torch.from_numpy(np.ascontiguousarray(z).view(np.float64).copy()).cuda()
+    vd, pd = cplx(vion), cplx(rhops)
+    cd = to_dev(psi)
+    ref, v = psi, sysm.v
+    for step in range(2):
+        e = ham.etrs_step(vb, cd.data_ptr(), basis.ngw, sysm.nst, sysm.occ, 1.0, dt, vd.data_ptr(),
+                          pd.data_ptr(), sysm.omega)
+        gpu_ctx.sync()
+        ref, v, eref = oracle_step(ref, v)
+        got = from_dev(cd, basis.ngw, sysm.nst)
+        assert rel_err(got, ref) <= FP64_TOL, step
+        assert all(abs(a - b) <= 1e-10 for a, b in zip(e, eref)), (step, e, eref)
+    ham.close()
+    basis.close()
+    vb.close()

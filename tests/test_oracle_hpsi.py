@@ -188,3 +188,33 @@ def test_subspace_diag_identities(real):
     assert np.max(np.abs(m - np.diag(w))) <= 1e-10 * np.max(np.abs(w))
     s = np.array([[gdot(cz[:, [i]], cz[:, [j]], real)[0] for j in range(5)] for i in range(5)])
     assert np.max(np.abs(s - np.eye(5))) <= 1e-12
+
+
+def test_etrs_step_splits_and_order():
+    """ETRS = exp(-i dt/2 H(t+dt)) exp(-i dt/2 H(t)) to O(dt^5): pins the half steps and
+    which Hamiltonian acts first (the two orders differ at O(dt^2) [H1, H2])."""
+    rng = np.random.default_rng(10)
+    n = 8
+    mk = lambda: (lambda a: 0.25 * (a + a.conj().T))(rng.standard_normal((n, n)) + 1j * rng.standard_normal((n, n)))
+    H1, H2 = mk(), mk()
+    psi = rng.standard_normal((n, 3)) + 1j * rng.standard_normal((n, 3))
+    dt = 0.02
+
+    def expm(H, t):
+        w, U = np.linalg.eigh(H)
+        return U @ np.diag(np.exp(-1j * t * w)) @ U.conj().T
+
+    seen = []
+
+    def h2_of(tilde):
+        seen.append(tilde)
+        return lambda w: H2 @ w
+    got, tilde = oh.etrs_step(lambda w: H1 @ w, h2_of, psi, dt)
+    lam = max(np.max(np.abs(np.linalg.eigvalsh(H1))), np.max(np.abs(np.linalg.eigvalsh(H2))))
+    bound = 3 * (dt * lam) ** 5
+    # H(t+dt) is built from psi~(t+dt) = exp(-i dt H(t)) psi
+    assert np.max(np.abs(seen[0] - expm(H1, dt) @ psi)) <= bound
+    ref = expm(H2, 0.5 * dt) @ expm(H1, 0.5 * dt) @ psi
+    assert np.max(np.abs(got - ref)) <= bound
+    wrong = expm(H1, 0.5 * dt) @ expm(H2, 0.5 * dt) @ psi
+    assert np.max(np.abs(got - wrong)) > 100 * bound
