@@ -157,6 +157,16 @@ int qbh_update_vhxc(qbh_basis vbasis, const double* rhor_dev, const double* rhog
 int qbh_exponential(qbh_ham ham, double* psi_dev, long ld, int nst, double dt, int order,
                     int flags);
 
+/* Electronic current (CurrentDensity::update_current, CurrentDensity.cc:39-106):
+ *   j_d(r) = -Im sum_n (weight/omega) occ_n conj(psi_n(r)) (FFT^-1[i (k+G)_d c_n])(r)
+ * over bands with positive factor (two-state SlaterDet::compute_density, SlaterDet.cc:
+ * 935-966), d = x, y, z with the basis' k+G.  j_dev (nullable) receives 3 grids (d-major,
+ * np0*np1*np2 each, overwritten); total (nullable, host, 3) receives
+ * (omega/N) sum_r j_d(r) + A_d * nelectrons (A nullable: no vector potential). */
+int qbh_current(qbh_basis basis, const double* psi_dev, long ld, int nst, const double* occ,
+                double weight, double omega, const double* A, double nelectrons, double* j_dev,
+                double* total);
+
 /* One ETRS step with a self-consistent H(t + dt) (ExponentialWavefunctionStepper::preupdate
  * and update, ExponentialWavefunctionStepper.cc:151-276, merge_exp_ false, ETRS not AETRS),
  * complex basis:

@@ -53,6 +53,8 @@ SIGNATURES = {
     "qbh_subspace_diag": (_c_int, [_vp, _vp, _c_long, _vp, _c_long, _c_int, _dp, _c_int]),
     "qbh_update_vhxc": (_c_int, [_vp, _vp, _vp, _vp, _vp, _vp, _c_double, _vp, _dp]),
     "qbh_exponential": (_c_int, [_vp, _vp, _c_long, _c_int, _c_double, _c_int, _c_int]),
+    "qbh_current": (_c_int, [_vp, _vp, _c_long, _c_int, _dp, _c_double, _c_double, _dp, _c_double,
+                             _vp, _dp]),
     "qbh_etrs_step": (_c_int, [_vp, _vp, _vp, _c_long, _c_int, _dp, _c_double, _c_double, _c_int,
                                _vp, _vp, _c_double, _dp, _c_int]),
 }
@@ -197,6 +199,19 @@ class Basis:
                                              rhops_g_ptr, float(omega), v_r_ptr,
                                              e.ctypes.data_as(_dp)), "qbh_update_vhxc")
         return tuple(e)
+
+    def current(self, psi_ptr, ld, nst, occ, weight, omega, A=None, nelectrons=0.0, j_ptr=None):
+        """CurrentDensity::update_current; returns the total current (3,) and fills the
+        current-density grids at j_ptr (3 x N doubles) when given."""
+        o = np.ascontiguousarray(occ, dtype=np.float64)
+        a = None if A is None else np.ascontiguousarray(A, dtype=np.float64)
+        t = np.zeros(3)
+        self.ctx.check(lib().qbh_current(self.h, psi_ptr, ld, nst, o.ctypes.data_as(_dp),
+                                         float(weight), float(omega),
+                                         None if a is None else a.ctypes.data_as(_dp),
+                                         float(nelectrons), j_ptr, t.ctypes.data_as(_dp)),
+                       "qbh_current")
+        return t
 
     def gram(self, psi_ptr, ld, nst):
         self.ctx.check(lib().qbh_gram(self.h, psi_ptr, ld, nst), "qbh_gram")

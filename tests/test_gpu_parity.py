@@ -424,3 +424,30 @@ def test_etrs_step_selfconsistent_si8c(gpu_ctx, si8c):
     ham.close()
     basis.close()
     vb.close()
+
+
+@pytest.mark.parametrize("which", ["si8c", "si8"])
+def test_current(gpu_ctx, which, request):
+    import torch
+    from oracle import hpsi as oh
+    from oracle.observables import current
+    sysm = request.getfixturevalue(which)
+    ob, maps, _ = oracle_for(sysm)
+    psi = oh.gram(sysm.psi, sysm.real)
+    occ = np.array(sysm.occ, dtype=np.float64)
+    occ[-1] = 0.0
+    A = np.array([0.01, -0.02, 0.03])
+    jref, tref = current(maps, psi, occ, 1.0, sysm.omega, A=A, nelectrons=float(occ.sum()))
+    basis = make_gpu_basis(gpu_ctx, sysm)
+    cd = to_dev(psi)
+    jd = torch.zeros(3 * maps.N, dtype=torch.float64, device="cuda")
+    tot = basis.current(cd.data_ptr(), basis.ngw, sysm.nst, occ, 1.0, sysm.omega, A=A,
+                        nelectrons=float(occ.sum()), j_ptr=jd.data_ptr())
+    gpu_ctx.sync()
+    basis.close()
+    j = jd.cpu().numpy().reshape(3, -1)
+    if sysm.real:   # Γ: real orbitals carry no paramagnetic current
+        assert np.max(np.abs(j)) <= 1e-12 and np.max(np.abs(tot - A * occ.sum())) <= 1e-12
+    else:
+        assert rel_err(j, jref) <= FP64_TOL
+        assert np.max(np.abs(tot - tref)) <= 1e-12 * np.max(np.abs(tref))
