@@ -384,6 +384,10 @@ extern "C" int qbh_hpsi(qbh_ham ham, const double* psi_dev, long ld, int nst, co
   if (!ham || !psi_dev || !hpsi_dev || nst <= 0) return QBH_ERR_ARG;
   Ham* h = &ham->h;
   hipSetDevice(h->ctx->device);
+  // complex elements: 16-byte aligned arrays (vector loads in the conversion kernels)
+  if ((reinterpret_cast<uintptr_t>(psi_dev) | reinterpret_cast<uintptr_t>(hpsi_dev)) & 15)
+    return fail(h->ctx, QBH_ERR_ARG, "psi/hpsi must be 16-byte aligned");
+  if (nst > 65535) return fail(h->ctx, QBH_ERR_ARG, "nst > 65535");
   return hpsi_impl(h, psi_dev, ld, nst, occ, hpsi_dev, ldh, flags, enl);
 }
 

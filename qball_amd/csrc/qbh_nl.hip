@@ -159,25 +159,50 @@ __global__ void k_sum_partials(int n, double* __restrict__ partial) {
 }
 
 // psi (FP64, ld complex) -> compact FP32 copy (2*ngw floats per band)
-__global__ void k_to_f32(int ngw, int nst, const double* __restrict__ psi, long ld,
-                         float* __restrict__ out) {
-  const long i = blockIdx.x * 256L + threadIdx.x;
-  const long total = 2L * ngw * nst;
-  if (i >= total) return;
-  const long n = i / (2L * ngw);
-  const long r = i - n * 2L * ngw;
-  out[i] = float(psi[n * 2 * ld + r]);
+// out (FP32 compact, 2*ngw floats per band) = psi (FP64, leading dimension ld complex);
+// grid (ceil(2*ngw/1024), nst): each thread converts 4 consecutive doubles
+__global__ void __launch_bounds__(256)
+k_to_f32(int ngw, const double* __restrict__ psi, long ld, float* __restrict__ out) {
+  const long n = blockIdx.y;
+  const int r = (blockIdx.x * 256 + threadIdx.x) * 4;
+  const int m = 2 * ngw;
+  if (r >= m) return;
+  const double* src = psi + n * 2 * ld + r;
+  float* dst = out + n * (long)m + r;
+  if (r + 4 <= m) {  // 2*ngw and 2*ld are even, so the pairs are 16-byte aligned
+    const double2 a = *reinterpret_cast<const double2*>(src);
+    const double2 b = *reinterpret_cast<const double2*>(src + 2);
+    float2 lo = make_float2(float(a.x), float(a.y)), hi = make_float2(float(b.x), float(b.y));
+    *reinterpret_cast<float2*>(dst) = lo;
+    *reinterpret_cast<float2*>(dst + 2) = hi;
+  } else {
+    for (int k = r; k < m; ++k) out[n * (long)m + k] = float(psi[n * 2 * ld + k]);
+  }
 }
 
-// hpsi (FP64) += tmp (FP32 compact)
-__global__ void k_add_f32(int ngw, int nst, const float* __restrict__ tmp, double* __restrict__ h,
-                          long ldh) {
-  const long i = blockIdx.x * 256L + threadIdx.x;
-  const long total = 2L * ngw * nst;
-  if (i >= total) return;
-  const long n = i / (2L * ngw);
-  const long r = i - n * 2L * ngw;
-  h[n * 2 * ldh + r] += double(tmp[i]);
+// hpsi (FP64, leading dimension ldh) += tmp (FP32 compact); grid as k_to_f32
+__global__ void __launch_bounds__(256)
+k_add_f32(int ngw, const float* __restrict__ tmp, double* __restrict__ h, long ldh) {
+  const long n = blockIdx.y;
+  const int r = (blockIdx.x * 256 + threadIdx.x) * 4;
+  const int m = 2 * ngw;
+  if (r >= m) return;
+  const float* src = tmp + n * (long)m + r;
+  double* dst = h + n * 2 * ldh + r;
+  if (r + 4 <= m) {
+    const float2 lo = *reinterpret_cast<const float2*>(src);
+    const float2 hi = *reinterpret_cast<const float2*>(src + 2);
+    double2 a = *reinterpret_cast<double2*>(dst);
+    double2 b = *reinterpret_cast<double2*>(dst + 2);
+    a.x += double(lo.x);
+    a.y += double(lo.y);
+    b.x += double(hi.x);
+    b.y += double(hi.y);
+    *reinterpret_cast<double2*>(dst) = a;
+    *reinterpret_cast<double2*>(dst + 2) = b;
+  } else {
+    for (int k = r; k < m; ++k) h[n * 2 * ldh + k] += double(tmp[n * (long)m + k]);
+  }
 }
 
 // ------------------------------------------------------------------------------------
@@ -381,7 +406,8 @@ int nl_apply(Ham* h, const double* psi, long ld, int nst, const double* occ_host
     }
     {
       TimedLaunch tl(c, "to_f32");
-      k_to_f32<<<(unsigned)((psz + 255) / 256), 256, 0, c->stream>>>(ngw, nst, psi, ld, p32);
+      k_to_f32<<<dim3((unsigned)((2 * ngw + 1023) / 1024), (unsigned)nst), 256, 0, c->stream>>>(
+          ngw, psi, ld, p32);
     }
     {
       TimedLaunch tl(c, "fnl_gemm");
@@ -408,7 +434,8 @@ int nl_apply(Ham* h, const double* psi, long ld, int nst, const double* occ_host
     }
     {
       TimedLaunch tl(c, "add_f32");
-      k_add_f32<<<(unsigned)((psz + 255) / 256), 256, 0, c->stream>>>(ngw, nst, t32, hpsi, ldh);
+      k_add_f32<<<dim3((unsigned)((2 * ngw + 1023) / 1024), (unsigned)nst), 256, 0, c->stream>>>(
+          ngw, t32, hpsi, ldh);
     }
   }
   hipError_t err = hipGetLastError();
