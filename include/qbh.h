@@ -39,6 +39,7 @@ extern "C" {
 typedef struct qbh_ctx_s* qbh_ctx;
 typedef struct qbh_basis_s* qbh_basis;
 typedef struct qbh_ham_s* qbh_ham;
+typedef struct qbh_td_s* qbh_td;
 
 /* ---- context ------------------------------------------------------------------------ */
 /* stream: a hipStream_t to run on, or NULL for a stream owned by the context. */
@@ -184,6 +185,35 @@ int qbh_etrs_step(qbh_ham ham, qbh_basis vbasis, double* psi_dev, long ld, int n
                   const double* occ, double weight, double dt, int order,
                   const double* vion_g_dev, const double* rhops_g_dev, double omega,
                   double* energies, int flags);
+
+/* ---- real-time TDDFT over many steps and its adjoint -----------------------------------
+ * The EhrenSampleStepper iteration (EhrenSampleStepper.cc:280-1152) for fixed ions, one
+ * spin and k-point, complex basis, per step n:
+ *   v_a = update_vhxc(rho(psi_n)) + v_ext                                  (:323-330)
+ *   preupdate under H[v_a, A_n]; A -> A_{n+1} (:477-480); update under
+ *   H[v_b, A_{n+1}] with v_b = update_vhxc(rho(psi~)) + v_ext    (qbh_etrs_step halves)
+ * and the objective J = sum_{n<=nsteps} w_n e.j(psi_n, A_n) over the total current of
+ * qbh_current (CurrentDensity.cc:39-106, j_d = -sum f_b (k+G)_d |c|^2 + A_d N_e).  The KB
+ * radial tables of ham stay as set (not re-tabled at |k+G-A|); the projector phases at
+ * k+G-A cancel in beta D beta^H, so A enters H through 0.5|k+G-A|^2.  No counterpart in the
+ * reference for the adjoint: qbh_td_backward is the exact derivative of this discrete map
+ * (dJ/dv_ext on the grid, dJ/dA_n), recomputing segments between checkpoints.
+ * ham (projectors set) and vbasis (complex density basis on the same grid) must outlive td;
+ * vion_g_dev / rhops_g_dev (nullable) are borrowed.  ham's v_r and A are restored on
+ * return. */
+int qbh_td_create(qbh_ham ham, qbh_basis vbasis, int nst, const double* occ, double weight,
+                  double omega, const double* vion_g_dev, const double* rhops_g_dev, qbh_td* out);
+int qbh_td_destroy(qbh_td td);
+/* psi0_dev (not modified); A: host 3*(nsteps+1); w: host nsteps+1; e: host 3 (direction);
+ * vext_dev: nullable, np0*np1*np2 doubles, borrowed until the matching backward;
+ * checkpoint: steps between stored psi; flags: QBH_FP32 selects the FP32-bulk Hψ.
+ * J receives the objective; psi_out_dev (nullable) psi_nsteps. */
+int qbh_td_forward(qbh_td td, const double* psi0_dev, long ld, int nsteps, double dt, int order,
+                   const double* A, const double* w, const double* e, const double* vext_dev,
+                   int checkpoint, int flags, double* J, double* psi_out_dev, long ldo);
+/* Gradients of the last forward's J: dvext_dev (nullable, device, N doubles, overwritten)
+ * and dA (nullable, host, 3*(nsteps+1)). */
+int qbh_td_backward(qbh_td td, double* dvext_dev, double* dA);
 
 #ifdef __cplusplus
 }

@@ -55,6 +55,12 @@ SIGNATURES = {
     "qbh_exponential": (_c_int, [_vp, _vp, _c_long, _c_int, _c_double, _c_int, _c_int]),
     "qbh_current": (_c_int, [_vp, _vp, _c_long, _c_int, _dp, _c_double, _c_double, _dp, _c_double,
                              _vp, _dp]),
+    "qbh_td_create": (_c_int, [_vp, _vp, _c_int, _dp, _c_double, _c_double, _vp, _vp,
+                               ctypes.POINTER(_vp)]),
+    "qbh_td_destroy": (_c_int, [_vp]),
+    "qbh_td_forward": (_c_int, [_vp, _vp, _c_long, _c_int, _c_double, _c_int, _dp, _dp, _dp, _vp,
+                                _c_int, _c_int, _dp, _vp, _c_long]),
+    "qbh_td_backward": (_c_int, [_vp, _vp, _dp]),
     "qbh_etrs_step": (_c_int, [_vp, _vp, _vp, _c_long, _c_int, _dp, _c_double, _c_double, _c_int,
                                _vp, _vp, _c_double, _dp, _c_int]),
 }
@@ -274,4 +280,46 @@ class Hamiltonian:
     def close(self):
         if self.h:
             lib().qbh_ham_destroy(self.h)
+            self.h = _vp()
+
+
+class TDPropagator:
+    """Real-time TDDFT over many ETRS iterations with the objective J = sum_n w_n e.j(psi_n,
+    A_n) and its adjoint (qbh_td_*).  ham and vbasis must outlive it; vion/rhops device
+    pointers are borrowed."""
+
+    def __init__(self, ham, vbasis, nst, occ, weight, omega, vion_g_ptr, rhops_g_ptr):
+        self.ctx = ham.ctx
+        self.h = _vp()
+        self._occ = np.ascontiguousarray(occ, dtype=np.float64)
+        self.ctx.check(lib().qbh_td_create(ham.h, vbasis.h, int(nst), self._occ.ctypes.data_as(_dp),
+                                           float(weight), float(omega), vion_g_ptr, rhops_g_ptr,
+                                           ctypes.byref(self.h)), "qbh_td_create")
+        self.nst = nst
+        self.nsteps = None
+
+    def forward(self, psi0_ptr, ld, nsteps, dt, A, w, e=(1.0, 0.0, 0.0), vext_ptr=None,
+                checkpoint=20, order=4, flags=0, psi_out_ptr=None, ldo=None):
+        A = np.ascontiguousarray(A, dtype=np.float64).reshape(nsteps + 1, 3)
+        w = np.ascontiguousarray(w, dtype=np.float64).reshape(nsteps + 1)
+        e = np.ascontiguousarray(e, dtype=np.float64).reshape(3)
+        J = ctypes.c_double(0.0)
+        self.ctx.check(lib().qbh_td_forward(self.h, psi0_ptr, int(ld), int(nsteps), float(dt),
+                                            int(order), A.ctypes.data_as(_dp), w.ctypes.data_as(_dp),
+                                            e.ctypes.data_as(_dp), vext_ptr, int(checkpoint),
+                                            int(flags), ctypes.byref(J), psi_out_ptr,
+                                            int(ldo if ldo is not None else ld)), "qbh_td_forward")
+        self.nsteps = nsteps
+        return J.value
+
+    def backward(self, dvext_ptr=None):
+        """Returns dJ/dA (nsteps+1, 3); fills dJ/dv_ext at dvext_ptr when given."""
+        dA = np.zeros((self.nsteps + 1, 3))
+        self.ctx.check(lib().qbh_td_backward(self.h, dvext_ptr, dA.ctypes.data_as(_dp)),
+                       "qbh_td_backward")
+        return dA
+
+    def close(self):
+        if self.h:
+            lib().qbh_td_destroy(self.h)
             self.h = _vp()

@@ -20,10 +20,6 @@ Ctx* ctx_of_basis(qbh_basis b);
 Ctx* ctx_of(qbh_ctx c);
 }  // namespace qbh
 
-struct qbh_ham_s {
-  Ham h;
-};
-
 static int fail(Ctx* c, int code, const std::string& msg) {
   c->last_error = msg;
   return code;
@@ -311,8 +307,8 @@ static int ensure_side(Ctx* c) {
   return 0;
 }
 
-static int hpsi_impl(Ham* h, const double* psi, long ld, int nst, const double* occ,
-                     double* hpsi, long ldh, int flags, double* enl) {
+int qbh::hpsi_impl(Ham* h, const double* psi, long ld, int nst, const double* occ,
+                   double* hpsi, long ldh, int flags, double* enl) {
   Ctx* c = h->ctx;
   Basis* b = h->b;
   const int ngw = b->g.ngw;
@@ -463,12 +459,14 @@ __global__ void k_zaxpy(long n, double ar, double ai, const double* __restrict__
 }
 
 // out = sum_{N<=order} (-i dt)^N/N! H^N in (ExponentialWavefunctionStepper::exponential,
-// ExponentialWavefunctionStepper.cc:51-148, num_exp = 1); out may alias in.
-static int expo_impl(Ham* h, const double* in, long ldin, double* outp, long ldout, int nst,
-                     double dt, int order, int flags) {
+// ExponentialWavefunctionStepper.cc:51-148, num_exp = 1); out may alias in.  krylov
+// (nullable): receives H^k in for k < order, block k at krylov + k*2*ngw*nst (ld = ngw).
+int qbh::expo_impl(Ham* h, const double* in, long ldin, double* outp, long ldout, int nst,
+                   double dt, int order, int flags, double* krylov) {
   Ctx* c = h->ctx;
   const int ngw = h->b->g.ngw;
   const size_t bytes = 2 * sizeof(double) * (size_t)ngw * nst;
+  const size_t blk = 2 * (size_t)ngw * nst;
   if (h->expo.size() != 3 || h->expo_bytes < bytes) {
     for (void* p : h->expo) hipFree(p);
     h->expo.assign(3, nullptr);
@@ -485,6 +483,9 @@ static int expo_impl(Ham* h, const double* in, long ldin, double* outp, long ldo
   // expwf = in (compact, ld = ngw)
   hipMemcpy2DAsync(expwf, 2 * sizeof(double) * ngw, in, 2 * sizeof(double) * ldin,
                    2 * sizeof(double) * ngw, nst, hipMemcpyDeviceToDevice, c->stream);
+  if (krylov)
+    hipMemcpy2DAsync(krylov, 2 * sizeof(double) * ngw, in, 2 * sizeof(double) * ldin,
+                     2 * sizeof(double) * ngw, nst, hipMemcpyDeviceToDevice, c->stream);
   std::vector<double> occ(nst, 0.0);
   double fr = 1.0, fi = 0.0;  // factor
   const double* x = in;
@@ -496,7 +497,7 @@ static int expo_impl(Ham* h, const double* in, long ldin, double* outp, long ldo
     const double nr = fi * m, ni = -fr * m;
     fr = nr;
     fi = ni;
-    double* out = bufs[(N - 1) & 1];
+    double* out = (krylov && N < order) ? krylov + N * blk : bufs[(N - 1) & 1];
     if (int r = hpsi_impl(h, x, ldx, nst, occ.data(), out, ngw, hflags, nullptr)) return r;
     const long n = (long)ngw * nst;
     k_zaxpy<<<(unsigned)((n + 255) / 256), 256, 0, c->stream>>>(n, fr, fi, out, ngw, expwf, ngw,
@@ -517,7 +518,7 @@ extern "C" int qbh_exponential(qbh_ham ham, double* psi_dev, long ld, int nst, d
   hipSetDevice(c->device);
   if (ld < h->b->g.ngw) return fail(c, QBH_ERR_ARG, "ld < ngw");
   if (h->b->g.real) return fail(c, QBH_ERR_ARG, "exponential needs a complex basis");
-  return expo_impl(h, psi_dev, ld, psi_dev, ld, nst, dt, order, flags);
+  return expo_impl(h, psi_dev, ld, psi_dev, ld, nst, dt, order, flags, nullptr);
 }
 
 // One ETRS step (ExponentialWavefunctionStepper::preupdate + update,
@@ -554,8 +555,8 @@ extern "C" int qbh_etrs_step(qbh_ham ham, qbh_basis vbasis, double* psi_dev, lon
   double* rho = tilde + 2 * (size_t)g.ngw * nst;
   double* rhog = rho + N;
   const int fl = flags & (QBH_ALL | QBH_FP32);
-  if (int r = expo_impl(h, psi_dev, ld, half, g.ngw, nst, 0.5 * dt, order, fl)) return r;
-  if (int r = expo_impl(h, half, g.ngw, tilde, g.ngw, nst, 0.5 * dt, order, fl)) return r;
+  if (int r = expo_impl(h, psi_dev, ld, half, g.ngw, nst, 0.5 * dt, order, fl, nullptr)) return r;
+  if (int r = expo_impl(h, half, g.ngw, tilde, g.ngw, nst, 0.5 * dt, order, fl, nullptr)) return r;
   fill0(c, rho, N);
   if (int r = qbh_density(reinterpret_cast<qbh_basis>(h->b_handle), tilde, g.ngw, nst, occ,
                           weight, omega, rho, flags & QBH_FP32))
@@ -564,5 +565,5 @@ extern "C" int qbh_etrs_step(qbh_ham ham, qbh_basis vbasis, double* psi_dev, lon
   if (int r = qbh_update_vhxc(vbasis, rho, rhog, rhog, vion_g_dev, rhops_g_dev, omega, h->v,
                               energies))
     return r;
-  return expo_impl(h, half, g.ngw, psi_dev, ld, nst, 0.5 * dt, order, fl);
+  return expo_impl(h, half, g.ngw, psi_dev, ld, nst, 0.5 * dt, order, fl, nullptr);
 }

@@ -45,6 +45,12 @@ struct Ham {
   size_t etrs_bytes = 0;
 };
 
+// Hψ of qbh_hpsi without argument checks beyond leading dimensions (qbh_api.hip)
+int hpsi_impl(Ham* h, const double* psi, long ld, int nst, const double* occ, double* hpsi,
+              long ldh, int flags, double* enl);
+// Taylor exponential of qbh_exponential; krylov (nullable) receives H^k in, k < order
+int expo_impl(Ham* h, const double* in, long ldin, double* outp, long ldout, int nst, double dt,
+              int order, int flags, double* krylov);
 int nl_apply(Ham* h, const double* psi, long ld, int nst, const double* occ, double* hpsi,
              long ldh, int fp32, double* enl);
 double nl_read_enl(Ham* h, hipStream_t s);
@@ -52,3 +58,8 @@ int blas_handle(Ctx* c, void** out);
 void* grow(void*& p, size_t& have, size_t need);
 
 }  // namespace qbh
+
+// the opaque qbh_ham handle of the C ABI
+struct qbh_ham_s {
+  qbh::Ham h;
+};
