@@ -40,6 +40,7 @@ typedef struct qbh_ctx_s* qbh_ctx;
 typedef struct qbh_basis_s* qbh_basis;
 typedef struct qbh_ham_s* qbh_ham;
 typedef struct qbh_td_s* qbh_td;
+typedef struct qbh_batch_s* qbh_batch;
 
 /* ---- context ------------------------------------------------------------------------ */
 /* stream: a hipStream_t to run on, or NULL for a stream owned by the context. */
@@ -185,6 +186,34 @@ int qbh_etrs_step(qbh_ham ham, qbh_basis vbasis, double* psi_dev, long ld, int n
                   const double* occ, double weight, double dt, int order,
                   const double* vion_g_dev, const double* rhops_g_dev, double omega,
                   double* energies, int flags);
+
+/* ---- launch tuning, autotuner, multi-cell graph (no counterpart in the reference) ------
+ * Knobs (every value computes the same Hψ; only speed differs):
+ *   "zy_reg"  1/0  register two-factor z/y transforms where instantiated (basis-wide)
+ *   "x_reg"   1/0  register x pass where instantiated (basis-wide)
+ *   "x_spw"   0..64 slots per x-pass workgroup, 0 = occupancy heuristic (basis-wide)
+ *   "radix"   0..2 Stockham radix plan variant for the other lengths (basis-wide)
+ *   "nsplit"  0..64 K slices of the projector contraction, 0 = heuristic
+ *   "overlap" -1/0/1 projector GEMMs concurrent with the transforms, -1 = context default */
+int qbh_ham_set_tuning(qbh_ham ham, const char* knob, int value);
+int qbh_ham_get_tuning(qbh_ham ham, const char* knob, int* value);
+/* Times candidate knob values for this Hψ shape (reps applications each, HIP events on the
+ * context's stream), keeps the fastest, and caches the choice per shape in the context
+ * (force != 0 re-tunes).  hpsi_dev is overwritten.  report (nullable) receives the timings. */
+int qbh_ham_autotune(qbh_ham ham, const double* psi_dev, long ld, int nst, const double* occ,
+                     double* hpsi_dev, long ldh, int flags, int reps, int force, char* report,
+                     int cap);
+/* Hψ of ncells cells (each: hams[i], psi_dev[i] with ld[i] and nst[i] bands, into
+ * hpsi_dev[i] with ldh[i]; flags as qbh_hpsi without ACCUMULATE) captured into one
+ * hipGraph over nbranch concurrent branches.  The context's stream must be a non-default
+ * stream (qbh_ctx_create with stream NULL).  The graph reads the Hamiltonians' current v_r,
+ * kinetic and projector tables at each launch; re-create it after changing a cell's shape,
+ * band count or projector set.  Cells sharing a Hamiltonian run on one branch. */
+int qbh_batch_create(qbh_ctx ctx, int ncells, const qbh_ham* hams, const double* const* psi_dev,
+                     const long* ld, const int* nst, double* const* hpsi_dev, const long* ldh,
+                     int flags, int nbranch, qbh_batch* out);
+int qbh_batch_launch(qbh_batch batch);
+int qbh_batch_destroy(qbh_batch batch);
 
 /* ---- real-time TDDFT over many steps and its adjoint -----------------------------------
  * The EhrenSampleStepper iteration (EhrenSampleStepper.cc:280-1152) for fixed ions, one

@@ -6,9 +6,9 @@ behind qball-shaped classes.  This Python package is a thin ctypes binding of th
 ABI, used by the tests and the benchmark; it refuses to run without the built library.
 """
 from .capi import (QBH_ALL, QBH_ACCUMULATE, QBH_FP32, QBH_KINETIC, QBH_LOCAL, QBH_NONLOCAL,
-                   Basis, Context, Hamiltonian, QbhError, TDPropagator, factorizable, lib,
+                   Basis, Batch, Context, Hamiltonian, QbhError, TDPropagator, factorizable, lib,
                    rods_build)
 
 __all__ = ["QBH_ALL", "QBH_ACCUMULATE", "QBH_FP32", "QBH_KINETIC", "QBH_LOCAL",
-           "QBH_NONLOCAL", "Basis", "Context", "Hamiltonian", "QbhError", "TDPropagator",
+           "QBH_NONLOCAL", "Basis", "Batch", "Context", "Hamiltonian", "QbhError", "TDPropagator",
            "factorizable", "lib", "rods_build"]

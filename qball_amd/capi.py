@@ -55,6 +55,15 @@ SIGNATURES = {
     "qbh_exponential": (_c_int, [_vp, _vp, _c_long, _c_int, _c_double, _c_int, _c_int]),
     "qbh_current": (_c_int, [_vp, _vp, _c_long, _c_int, _dp, _c_double, _c_double, _dp, _c_double,
                              _vp, _dp]),
+    "qbh_ham_set_tuning": (_c_int, [_vp, ctypes.c_char_p, _c_int]),
+    "qbh_ham_get_tuning": (_c_int, [_vp, ctypes.c_char_p, _ip]),
+    "qbh_ham_autotune": (_c_int, [_vp, _vp, _c_long, _c_int, _dp, _vp, _c_long, _c_int, _c_int,
+                                  _c_int, ctypes.c_char_p, _c_int]),
+    "qbh_batch_create": (_c_int, [_vp, _c_int, ctypes.POINTER(_vp), ctypes.POINTER(_vp),
+                                  ctypes.POINTER(_c_long), _ip, ctypes.POINTER(_vp),
+                                  ctypes.POINTER(_c_long), _c_int, _c_int, ctypes.POINTER(_vp)]),
+    "qbh_batch_launch": (_c_int, [_vp]),
+    "qbh_batch_destroy": (_c_int, [_vp]),
     "qbh_td_create": (_c_int, [_vp, _vp, _c_int, _dp, _c_double, _c_double, _vp, _vp,
                                ctypes.POINTER(_vp)]),
     "qbh_td_destroy": (_c_int, [_vp]),
@@ -261,6 +270,25 @@ class Hamiltonian:
                                       ctypes.byref(enl) if want_enl else None), "qbh_hpsi")
         return enl.value if want_enl else None
 
+    def set_tuning(self, knob, value):
+        self.ctx.check(lib().qbh_ham_set_tuning(self.h, knob.encode(), int(value)),
+                       "qbh_ham_set_tuning")
+
+    def get_tuning(self, knob):
+        v = ctypes.c_int(0)
+        self.ctx.check(lib().qbh_ham_get_tuning(self.h, knob.encode(), ctypes.byref(v)),
+                       "qbh_ham_get_tuning")
+        return v.value
+
+    def autotune(self, psi_ptr, ld, nst, occ, hpsi_ptr, ldh, flags=QBH_ALL, reps=3, force=False):
+        """Times the launch choices for this shape and keeps the fastest; returns the report."""
+        o_, op = _da(occ)
+        buf = ctypes.create_string_buffer(8192)
+        self.ctx.check(lib().qbh_ham_autotune(self.h, psi_ptr, ld, nst, op, hpsi_ptr, ldh, flags,
+                                              int(reps), int(bool(force)), buf, len(buf)),
+                       "qbh_ham_autotune")
+        return buf.value.decode()
+
     def exponential(self, psi_ptr, ld, nst, dt, order=4, flags=QBH_ALL):
         self.ctx.check(lib().qbh_exponential(self.h, psi_ptr, ld, nst, float(dt), int(order),
                                              flags), "qbh_exponential")
@@ -322,4 +350,31 @@ class TDPropagator:
     def close(self):
         if self.h:
             lib().qbh_td_destroy(self.h)
+            self.h = _vp()
+
+
+class Batch:
+    """Hψ of several cells captured into one hipGraph (qbh_batch_*).  cells: list of
+    (Hamiltonian, psi_ptr, ld, nst, hpsi_ptr, ldh); all on ctx, whose stream must be a
+    non-default stream."""
+
+    def __init__(self, ctx, cells, flags=QBH_ALL, nbranch=4):
+        n = len(cells)
+        self.ctx = ctx
+        self.h = _vp()
+        hams = (_vp * n)(*[c[0].h for c in cells])
+        psis = (_vp * n)(*[c[1] for c in cells])
+        lds = (_c_long * n)(*[c[2] for c in cells])
+        nsts = (ctypes.c_int * n)(*[c[3] for c in cells])
+        outs = (_vp * n)(*[c[4] for c in cells])
+        ldhs = (_c_long * n)(*[c[5] for c in cells])
+        ctx.check(lib().qbh_batch_create(ctx.h, n, hams, psis, lds, nsts, outs, ldhs, flags,
+                                         int(nbranch), ctypes.byref(self.h)), "qbh_batch_create")
+
+    def launch(self):
+        self.ctx.check(lib().qbh_batch_launch(self.h), "qbh_batch_launch")
+
+    def close(self):
+        if self.h:
+            lib().qbh_batch_destroy(self.h)
             self.h = _vp()

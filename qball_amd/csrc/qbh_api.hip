@@ -329,7 +329,7 @@ int qbh::hpsi_impl(Ham* h, const double* psi, long ld, int nst, const double* oc
   // kinetic and local terms to hpsi, waits for them.
   hipEvent_t join = nullptr;
   if (do_nl) {
-    const bool overlap = do_loc && c->overlap;
+    const bool overlap = do_loc && (h->overlap >= 0 ? h->overlap != 0 : c->overlap);
     if (overlap) {
       if (int r = ensure_side(c)) return r;
       hipEventRecord(c->ev_fork, main);

@@ -166,9 +166,11 @@ extern "C" int qbh_ctx_timing_report(qbh_ctx ctx, char* buf, int cap) {
 }
 
 // ------------------------------------------------------------------------------------
-// radix plan for one axis: powers of two as 8s and 4s (a single 2 only when the
-// exponent is 1), then 3, 5, 7, 11
-static bool make_axis(int n, AxisPlan& p) {
+// Stockham radix plan for one axis.  Variant 0: powers of two as 8s and 4s (a single 2
+// only when the exponent is 1), then 3, 5, 7, 11; variant 1: powers of two as 4s (and one
+// 2); variant 2: the odd radices first, then the powers of two of variant 0.  Every
+// variant is the same transform; the choice only changes the pass count and strides.
+bool qbh::make_axis_variant(int n, int variant, AxisPlan& p) {
   p.n = n;
   p.npass = 0;
   int m = n, e = 0;
@@ -181,26 +183,35 @@ static bool make_axis(int n, AxisPlan& p) {
     p.radix[p.npass++] = r;
     return true;
   };
-  if (e == 1) {
-    if (!push(2)) return false;
-  } else {
+  auto odd = [&]() {
+    int q = m;
+    for (int r : {3, 5, 7, 11})
+      while (q % r == 0) {
+        q /= r;
+        if (!push(r)) return false;
+      }
+    return q == 1;
+  };
+  auto twos = [&]() {
+    if (variant == 1) {
+      for (int i = 0; i < e / 2; ++i) if (!push(4)) return false;
+      if (e % 2 && !push(2)) return false;
+      return true;
+    }
+    if (e == 1) return push(2);
     int e8 = e / 3, rem = e % 3;
     if (rem == 1 && e8 > 0) {  // 2^(3k+1) = 8^(k-1) * 4 * 4
       e8 -= 1;
       for (int i = 0; i < e8; ++i) if (!push(8)) return false;
-      if (!push(4) || !push(4)) return false;
-    } else {
-      for (int i = 0; i < e8; ++i) if (!push(8)) return false;
-      if (rem == 2 && !push(4)) return false;
-      if (rem == 1 && !push(2)) return false;
+      return push(4) && push(4);
     }
-  }
-  for (int r : {3, 5, 7, 11})
-    while (m % r == 0) {
-      m /= r;
-      if (!push(r)) return false;
-    }
-  return m == 1;
+    for (int i = 0; i < e8; ++i) if (!push(8)) return false;
+    if (rem == 2 && !push(4)) return false;
+    if (rem == 1 && !push(2)) return false;
+    return true;
+  };
+  if (variant == 2) return odd() && twos();
+  return twos() && odd();
 }
 
 template <typename X>
@@ -348,7 +359,7 @@ extern "C" int qbh_basis_create(qbh_ctx ctx, int real, int nrods, const int* rod
   // twiddles exp(+2 pi i m/n) in extended precision
   for (int ax = 0; ax < 3; ++ax) {
     const int n = nps[ax];
-    if (!make_axis(n, g.ax[ax])) return fail("FFT length has unsupported factors");
+    if (!make_axis_variant(n, 0, g.ax[ax])) return fail("FFT length has unsupported factors");
     std::vector<cplx<double>> t64(n);
     std::vector<cplx<float>> t32(n);
     const long double pi = 3.141592653589793238462643383279502884L;

@@ -801,10 +801,11 @@ static int check_launch(Ctx* ctx, const char* what) {
   } while (0)
 
 // per-length dispatch for the z and y passes: F1*F2 = n selects the two-factor register
-// transform; F1 = 0 the LDS Stockham passes
+// transform; F1 = 0 the LDS Stockham passes (also for every length when the basis' tuning
+// turns the register transforms off)
 #define QBH_SIZE_DISPATCH(n, CALL)                       \
   do {                                                   \
-    switch (n) {                                         \
+    switch (b->tune.zy_reg ? (n) : 0) {                                         \
       case 28: {                                         \
         constexpr int NM = 144, F1 = 7, F2 = 4;          \
         CALL;                                            \
@@ -925,6 +926,8 @@ static bool has_x2(int n) {
   return f1 > 0;
 }
 
+bool register_edge(int n) { return has_x2(n); }
+
 // two-factor register x pass for the grid edges it is instantiated for
 template <typename T, int OP>
 static bool launch_x2(Ctx* ctx, const GridDev& g, dim3 grid, size_t lds, cplx<T>* t2, int nslots,
@@ -955,13 +958,15 @@ static int xpass_t(Ctx* ctx, Basis* b, const PipelineArgs& a, const double* in, 
                    const double* dens, double scale, int cplx_density, const char* name) {
   const GridDev& g = b->g;
   auto* t2 = static_cast<cplx<T>*>(a.t2);
-  const bool reg = (OP == X_VMUL || OP == X_DENSITY) && has_x2(g.np0);
+  const bool reg = (OP == X_VMUL || OP == X_DENSITY) && has_x2(g.np0) && b->tune.x_reg;
   const int rows = reg ? Cfg<T>::XR : Cfg<T>::LPW;  // rows per wavefront
   const int yb = rows * NWAVE;
   const int nyb = (g.np1 + yb - 1) / yb;
   const size_t lds =
       sizeof(cplx<T>) * ((size_t)g.np0 + (size_t)NWAVE * rows * (reg ? lds_row(g.np0) : g.np0));
-  int spw = (OP == X_DENSITY) ? a.nslots : x_slots_per_wg(g, a.nslots, yb);
+  int spw = (OP == X_DENSITY) ? a.nslots
+            : b->tune.x_spw > 0 ? std::min(b->tune.x_spw, a.nslots)
+                                : x_slots_per_wg(g, a.nslots, yb);
   dim3 grid(g.np2 * nyb, (a.nslots + spw - 1) / spw);
   {
     TimedLaunch tl(ctx, name);

@@ -41,6 +41,9 @@ struct Ham {
   std::vector<double> occ_host;   // occupations currently on the device
   std::vector<void*> expo;        // exponential buffers
   size_t expo_bytes = 0;
+  // tuning (qbh_ham_set_tuning / qbh_ham_autotune): 0 / -1 = built-in choice
+  int nsplit = 0;    // K slices of the projector contraction
+  int overlap = -1;  // projector GEMMs on the side stream (1), after the transforms (0)
   void* etrs = nullptr;           // ETRS step: psi(t+dt/2), psi~(t+dt), rho, rhog
   size_t etrs_bytes = 0;
 };

@@ -64,7 +64,17 @@ struct Basis {
   double* kpg2 = nullptr;
   double* kpgx = nullptr;
   int max_lines_z = 0, max_lines_y = 0;
+  // launch choices (qbh_ham_set_tuning / qbh_ham_autotune)
+  struct Tune {
+    int zy_reg = 1;  // register two-factor z/y transforms where instantiated (else Stockham)
+    int x_reg = 1;   // register x pass where instantiated
+    int x_spw = 0;   // slots per x-pass workgroup (0 = occupancy heuristic)
+    int radix = 0;   // Stockham radix plan variant (0: 8s first, 1: 4s, 2: odd radices first)
+  } tune;
 };
+
+// Stockham radix plan of length n for a plan variant (qbh_basis.hip)
+bool make_axis_variant(int n, int variant, AxisPlan& p);
 
 // Kernel timing registry (optional; enabled per context).  Each launch of a timed
 // kernel is bracketed by two events on the context's stream; totals are folded in when
@@ -95,6 +105,8 @@ struct Ctx {
   void* ws = nullptr;
   size_t ws_bytes = 0;
   size_t ws_cap = size_t(8) << 30;  // cap on pipeline workspace
+  // autotuner results per problem shape (key -> knob values)
+  std::map<std::string, std::vector<int>> tune_cache;
 };
 
 // Records the named kernel's launch window when timing is enabled.
@@ -136,5 +148,7 @@ int pipeline_x_load(Ctx* ctx, Basis* b, const PipelineArgs& a, const double* f, 
 int pipeline_x_density(Ctx* ctx, Basis* b, const PipelineArgs& a, const double* facs,
                        double* rho);
 size_t slot_bytes(const Basis* b, int prec);
+// a two-factor register transform is instantiated for edge length n
+bool register_edge(int n);
 
 }  // namespace qbh

@@ -299,11 +299,10 @@ static int fnl_splits(int P, int nst, long K) {
 // Kelem = contraction length in elements (2*ngw reals or ngw complex).
 template <typename T>
 static int fnl_gemm(Ctx* c, rocblas_handle bh, bool cplx, int P, int nst, long Kelem, const T* anl,
-                    long lda, const T* psi, long ldb, T* F, int nsplit) {
+                    long lda, const T* psi, long ldb, T* F, long kc) {
   const rocblas_operation ta =
       cplx ? rocblas_operation_conjugate_transpose : rocblas_operation_transpose;
   const int e = cplx ? 2 : 1;  // T per element
-  const long kc = (Kelem + nsplit - 1) / nsplit;
   const long nfull = Kelem / kc;
   const long rem = Kelem - nfull * kc;
   const long pstride = (long)P * nst * e;
@@ -364,7 +363,11 @@ int nl_apply(Ham* h, const double* psi, long ld, int nst, const double* occ_host
 
   const long Kelem = real ? 2L * ngw : (long)ngw;  // contraction length in elements
   const int e = cplx ? 2 : 1;
-  const int nsplit = fnl_splits(P, nst, Kelem);
+  // K slices of kc elements; the slice count actually written (ceil(K / kc)) can be below
+  // the requested split count, and the finish kernel sums exactly that many partials
+  const int want = h->nsplit > 0 ? h->nsplit : fnl_splits(P, nst, Kelem);
+  const long kc = (Kelem + want - 1) / want;
+  const int nsplit = (int)((Kelem + kc - 1) / kc);
   const long pstride = (long)P * nst * e;
   const size_t fsz = (size_t)pstride * nsplit;
   if (!fp32) {
@@ -374,7 +377,7 @@ int nl_apply(Ham* h, const double* psi, long ld, int nst, const double* occ_host
     {
       TimedLaunch tl(c, "fnl_gemm");
       if (int r = fnl_gemm<double>(c, bh, cplx, P, nst, Kelem, anl, Kelem, psi,
-                                   real ? 2 * ld : ld, F, nsplit))
+                                   real ? 2 * ld : ld, F, kc))
         return r;
     }
     {
@@ -411,7 +414,7 @@ int nl_apply(Ham* h, const double* psi, long ld, int nst, const double* occ_host
     }
     {
       TimedLaunch tl(c, "fnl_gemm");
-      if (int r = fnl_gemm<float>(c, bh, cplx, P, nst, Kelem, anl, Kelem, p32, Kelem, F, nsplit))
+      if (int r = fnl_gemm<float>(c, bh, cplx, P, nst, Kelem, anl, Kelem, p32, Kelem, F, kc))
         return r;
     }
     {

@@ -1,0 +1,136 @@
+"""Launch knobs, the autotuner and the multi-cell Hψ graph: every choice keeps parity with
+the oracle, and graph replays match eager Hψ."""
+import numpy as np
+import pytest
+
+from helpers import from_dev, oracle_for, real_to_dev, rel_err, to_dev
+
+pytestmark = pytest.mark.gpu
+TOL = 1e-12
+
+
+def _cell(ctx, which, grid=None, seed=0):
+    """(system, oracle maps, projectors, gpu basis, gpu ham) for si8 (Γ) or si8c (complex)."""
+    from oracle.basis import Basis as OB, FFTMaps, UnitCell
+    from qball_amd import Basis, Hamiltonian
+    from qball_amd.workload import make_system
+    sysm = make_system(ncell=1, ecut_ry=25.0, nbands=7 if which == "si8" else 6, seed=seed,
+                       complex_basis=(which == "si8c"))
+    grid = tuple(grid or sysm.np)
+    ob, _, sp = oracle_for(sysm)
+    maps = FFTMaps(ob, *grid)
+    r = sysm.rods
+    basis = Basis(ctx, sysm.real, r["rod_h"], r["rod_k"], r["rod_lmin"], r["rod_size"], grid, sysm.b)
+    rng = np.random.default_rng(seed + 100)
+    v = rng.standard_normal(maps.N) * 0.3
+    ham = Hamiltonian(ctx, basis)
+    vd = real_to_dev(v)
+    ham.set_vloc(vd.data_ptr())
+    ham._v = vd
+    ham.set_projectors([sysm.na], [len(sysm.lproj)], sysm.lproj, sysm.wt, sysm.twnl.ravel(),
+                       sysm.tau.ravel(), sysm.omega)
+    return dict(sysm=sysm, maps=maps, sp=sp, basis=basis, ham=ham, v=v)
+
+
+def _ref(c):
+    from oracle import hpsi as oh
+    s = c["sysm"]
+    return oh.hpsi(c["maps"], s.psi, c["v"], [c["sp"]], s.occ, s.omega)[0]
+
+
+def _run(ctx, c, flags=0x7):
+    import torch
+    s = c["sysm"]
+    cd = to_dev(s.psi)
+    hd = torch.zeros_like(cd)
+    c["ham"].hpsi(cd.data_ptr(), c["basis"].ngw, s.nst, s.occ, hd.data_ptr(), c["basis"].ngw,
+                  flags=flags)
+    ctx.sync()
+    return from_dev(hd, c["basis"].ngw, s.nst)
+
+
+@pytest.mark.parametrize("which,grid", [("si8", None), ("si8", (36, 36, 72)), ("si8c", (36, 28, 40))])
+def test_every_knob_value_keeps_parity(gpu_ctx, which, grid):
+    c = _cell(gpu_ctx, which, grid)
+    ref = _ref(c)
+    ham = c["ham"]
+    cases = [("radix", 1), ("radix", 2), ("radix", 0), ("x_spw", 1), ("x_spw", 3), ("x_spw", 16),
+             ("x_spw", 0), ("zy_reg", 0), ("x_reg", 0), ("zy_reg", 1), ("x_reg", 1),
+             ("nsplit", 1), ("nsplit", 3), ("nsplit", 7), ("nsplit", 32), ("nsplit", 0),
+             ("overlap", 0), ("overlap", 1), ("overlap", -1)]
+    for knob, val in cases:
+        ham.set_tuning(knob, val)
+        assert ham.get_tuning(knob) == val
+        got = _run(gpu_ctx, c)
+        assert rel_err(got, ref) <= TOL, (knob, val)
+    ham.close()
+    c["basis"].close()
+
+
+def test_autotune_keeps_parity_and_caches(gpu_ctx):
+    import torch
+    c = _cell(gpu_ctx, "si8", (36, 36, 72))
+    s = c["sysm"]
+    cd = to_dev(s.psi)
+    hd = torch.zeros_like(cd)
+    rep = c["ham"].autotune(cd.data_ptr(), c["basis"].ngw, s.nst, s.occ, hd.data_ptr(),
+                            c["basis"].ngw, reps=2)
+    print(rep)
+    assert "->" in rep
+    assert rel_err(_run(gpu_ctx, c), _ref(c)) <= TOL
+    rep2 = c["ham"].autotune(cd.data_ptr(), c["basis"].ngw, s.nst, s.occ, hd.data_ptr(),
+                             c["basis"].ngw, reps=2)
+    assert rep2.startswith("cached")
+    c["ham"].close()
+    c["basis"].close()
+
+
+@pytest.mark.parametrize("flags", [0x7, 0x17])
+def test_batch_graph_replays_match_eager(flags):
+    """Three uneven cells in one graph over two branches; each replay (with eager work and
+    synchronisation in between) equals the eager Hψ of every cell."""
+    import torch
+    from qball_amd import Batch, Context
+    ctx = Context(0, None)           # a stream of its own: capturable
+    cells = [_cell(ctx, "si8", None, 1), _cell(ctx, "si8c", (36, 28, 40), 2),
+             _cell(ctx, "si8", (36, 36, 72), 3)]
+    eager, ins, outs = [], [], []
+    for c in cells:
+        s = c["sysm"]
+        cd = to_dev(s.psi)
+        torch.cuda.synchronize()
+        hd = torch.zeros_like(cd)
+        c["ham"].hpsi(cd.data_ptr(), c["basis"].ngw, s.nst, s.occ, hd.data_ptr(), c["basis"].ngw,
+                      flags=flags)
+        ctx.sync()
+        eager.append(from_dev(hd, c["basis"].ngw, s.nst))
+        ins.append(cd)
+        outs.append(torch.zeros_like(cd))
+    if flags == 0x7:
+        for c, e in zip(cells, eager):
+            assert rel_err(e, _ref(c)) <= TOL
+    torch.cuda.synchronize()
+    batch = Batch(ctx, [(c["ham"], i.data_ptr(), c["basis"].ngw, c["sysm"].nst, o.data_ptr(),
+                         c["basis"].ngw) for c, i, o in zip(cells, ins, outs)], flags=flags,
+                  nbranch=2)
+    ctx.sync()
+    scratch = torch.zeros_like(ins[0])
+    for rep in range(3):
+        for o in outs:
+            o.fill_(7.0)
+        torch.cuda.synchronize()
+        batch.launch()
+        ctx.sync()
+        for c, o, e in zip(cells, outs, eager):
+            got = from_dev(o, c["basis"].ngw, c["sysm"].nst)
+            assert np.max(np.abs(got - e)) <= 1e-14 * np.max(np.abs(e)), rep
+        # eager work between replays
+        c0 = cells[0]
+        c0["ham"].hpsi(ins[0].data_ptr(), c0["basis"].ngw, c0["sysm"].nst, c0["sysm"].occ,
+                       scratch.data_ptr(), c0["basis"].ngw, flags=flags)
+        ctx.sync()
+    batch.close()
+    for c in cells:
+        c["ham"].close()
+        c["basis"].close()
+    ctx.close()
