@@ -34,6 +34,8 @@ def parse():
     p.add_argument("--no-cpu-baseline", action="store_true")
     p.add_argument("--cpu-sample-pairs", type=int, default=4)
     p.add_argument("--no-fp32", action="store_true")
+    p.add_argument("--no-batch", action="store_true", help="skip the 12-cell graph section")
+    p.add_argument("--batch-reps", type=int, default=10)
     return p.parse_args()
 
 
@@ -190,6 +192,10 @@ def main():
                 "schedule": sched32, "kernels": k32}
         del h64
 
+    batch = None
+    if not args.no_batch:
+        batch = batch_bench(args)
+
     # CPU baseline: the oracle (numpy restatement) on a bounded sample, 1 thread
     cpu = None
     if not args.no_cpu_baseline:
@@ -214,6 +220,7 @@ def main():
         "roofline": roof,
         "cpu_baseline": cpu,
         "fp32_bulk": fp32,
+        "batching": batch,
         "schedule": sched64,
         "kernels": per_kernel,
         "band_energy_Ha": e64,
@@ -223,6 +230,76 @@ def main():
     ham.close()
     basis.close()
     ctx.close()
+
+
+def batch_bench(args):
+    """12 uneven cells (cubic boxes 12-28 bohr, 32-48 atoms, 64-320 bands, ecut 30 Ry, Γ),
+    FP32-bulk Hψ after FP64 Gram orthonormalisation: eager one-by-one with built-in launch
+    choices, eager after the autotuner, and all cells in one hipGraph launch."""
+    import torch
+    from qball_amd import QBH_ALL, QBH_FP32, Basis, Batch, Context, Hamiltonian
+    from qball_amd.workload import batch_workload, make_batch_cell
+    flags = QBH_ALL | QBH_FP32
+    ctx = Context(0, None)
+    cells = []
+    for p in batch_workload():
+        s = make_batch_cell(**p)
+        r = s.rods
+        basis = Basis(ctx, True, r["rod_h"], r["rod_k"], r["rod_lmin"], r["rod_size"], s.np, s.b)
+        ham = Hamiltonian(ctx, basis)
+        v = torch.from_numpy(s.v.copy()).cuda()
+        ham.set_vloc(v.data_ptr())
+        ham.set_projectors([s.na], [4], s.lproj, s.wt, s.twnl.ravel(), s.tau.ravel(), s.omega)
+        psi = torch.from_numpy(np.ascontiguousarray(s.psi.T).view(np.float64).ravel().copy()).cuda()
+        torch.cuda.synchronize()
+        basis.gram(psi.data_ptr(), s.ngw, s.nst)
+        out = torch.zeros_like(psi)
+        cells.append(dict(s=s, basis=basis, ham=ham, v=v, psi=psi, out=out))
+        del s.psi
+    ctx.sync()
+    nb_total = sum(c["s"].nst for c in cells)
+
+    def eager():
+        for c in cells:
+            c["ham"].hpsi(c["psi"].data_ptr(), c["s"].ngw, c["s"].nst, c["s"].occ,
+                          c["out"].data_ptr(), c["s"].ngw, flags=flags)
+
+    def timed(fn, reps):
+        fn()
+        ctx.sync()
+        t = time.perf_counter()
+        for _ in range(reps):
+            fn()
+        ctx.sync()
+        return (time.perf_counter() - t) / reps
+
+    t_eager = timed(eager, args.batch_reps)
+    tuned = []
+    for c in cells:
+        c["ham"].autotune(c["psi"].data_ptr(), c["s"].ngw, c["s"].nst, c["s"].occ,
+                          c["out"].data_ptr(), c["s"].ngw, flags=flags, reps=3)
+        tuned.append({k: c["ham"].get_tuning(k) for k in
+                      ("zy_reg", "x_reg", "x_spw", "radix", "nsplit", "overlap")})
+    t_tuned = timed(eager, args.batch_reps)
+    ref = [c["out"].clone() for c in cells]
+    g = Batch(ctx, [(c["ham"], c["psi"].data_ptr(), c["s"].ngw, c["s"].nst, c["out"].data_ptr(),
+                     c["s"].ngw) for c in cells], flags=flags, nbranch=4)
+    t_graph = timed(g.launch, args.batch_reps)
+    diff = max(float(torch.max(torch.abs(c["out"] - r)) / torch.max(torch.abs(r)))
+               for c, r in zip(cells, ref))
+    out = {"cells": len(cells), "bands_total": nb_total, "dtype": "fp32-bulk",
+           "value": nb_total / t_graph, "unit": "band-applications/s",
+           "graph_ms": t_graph * 1e3, "eager_default_ms": t_eager * 1e3,
+           "eager_tuned_ms": t_tuned * 1e3, "graph_vs_eager_max_rel_diff": diff,
+           "shapes": [{"box": round(c["s"].a[0, 0], 2), "atoms": c["s"].na, "bands": c["s"].nst,
+                       "pw": c["s"].ngw, "grid": c["s"].np[0]} for c in cells],
+           "tuning": tuned}
+    g.close()
+    for c in cells:
+        c["ham"].close()
+        c["basis"].close()
+    ctx.close()
+    return out
 
 
 def cpu_baseline(sysm, npairs, psi_dev, ngw, nst):

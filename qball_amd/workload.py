@@ -96,7 +96,14 @@ def make_system(ncell=2, ecut_ry=25.0, nbands=None, seed=0, complex_basis=False,
     s.occ = np.full(s.nst, 2.0)
     s.seed = seed
     rng = np.random.default_rng(seed)
-    # projectors: s + p, 4 per atom
+    _fill_tables(s, rng, alat)
+    if generate_psi:
+        s.psi = random_psi(s.ngw, s.nst, s.g2, rng, real=s.real)
+    return s
+
+
+def _fill_tables(s, rng, box):
+    """s + p projector tables (4 per atom) and the smooth random v_loc of a cubic box."""
     g = np.sqrt(s.g2)
     rc = 1.0
     vs = 2.0 * np.exp(-0.5 * (g * rc) ** 2)
@@ -115,13 +122,67 @@ def make_system(ncell=2, ecut_ry=25.0, nbands=None, seed=0, complex_basis=False,
     fz = np.fft.fftfreq(n2) * n2
     fy = np.fft.fftfreq(n1) * n1
     fx = np.fft.fftfreq(n0) * n0
-    bb = 2 * math.pi / alat
+    bb = 2 * math.pi / box
     G2 = (fz[:, None, None] * bb) ** 2 + (fy[None, :, None] * bb) ** 2 + (fx[None, None, :] * bb) ** 2
     v = np.real(np.fft.ifftn(np.fft.fftn(noise) * np.exp(-0.5 * G2)))
     s.v = (v / np.abs(v).max()).ravel().astype(np.float64)
+
+
+def smooth5(n):
+    """Smallest even 5-smooth integer >= n."""
+    m = max(2, n + (n % 2))
+    while True:
+        q = m
+        for p in (2, 3, 5):
+            while q % p == 0:
+                q //= p
+        if q == 1:
+            return m
+        m += 2
+
+
+def make_batch_cell(box, natoms, ecut_ry, nbands, seed, generate_psi=True):
+    """One cell of the multi-cell workload: a cubic box of edge `box` bohr holding `natoms`
+    atoms at jittered diamond sites (first natoms sites of an m^3 diamond supercell scaled
+    to the box, jitter N(0, 0.1 bohr)), Γ, ecut_ry, nbands bands, s + p projectors; the
+    Hψ grid is the density-grid rule rounded up to 5-smooth lengths."""
+    s = System()
+    rng = np.random.default_rng(seed)
+    s.a = np.diag([box, box, box])
+    s.ecut = ecut_ry * RY
+    s.omega, s.b = cell_vectors(s.a)
+    s.real = True
+    s.rods = capi.rods_build(s.a, s.ecut, real=True)
+    s.np = tuple(smooth5(n) for n in density_grid_np(s.a, s.ecut))
+    s.idx, s.kpg = plane_waves(s.rods, s.b)
+    s.ngw = s.idx.shape[0]
+    s.g2 = (s.kpg ** 2).sum(axis=1)
+    m = 1
+    while 8 * m ** 3 < natoms:
+        m += 1
+    sites = diamond_positions(m, alat0=box / m)
+    s.tau = sites[:natoms] + rng.normal(0.0, 0.1, (natoms, 3))
+    s.na = natoms
+    s.nst = nbands
+    s.occ = np.full(nbands, 2.0)
+    s.seed = seed
+    _fill_tables(s, rng, box)
     if generate_psi:
-        s.psi = random_psi(s.ngw, s.nst, s.g2, rng, real=s.real)
+        s.psi = random_psi(s.ngw, s.nst, s.g2, rng, real=True)
     return s
+
+
+def batch_workload(ncells=12, seed=0, ecut_ry=30.0):
+    """Parameters of the uneven multi-cell workload: per cell (box in [12, 28] bohr,
+    natoms in [32, 48], nbands in [64, 320]), uniform draws from one seeded generator."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for i in range(ncells):
+        box = float(rng.uniform(12.0, 28.0))
+        natoms = int(rng.integers(32, 49))
+        nbands = int(rng.integers(64, 321))
+        out.append(dict(box=box, natoms=natoms, nbands=nbands, ecut_ry=ecut_ry, seed=seed * 100 + i))
+    return out
 
 
 def random_psi(ngw, nst, g2, rng, real=True):
