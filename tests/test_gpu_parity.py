@@ -451,3 +451,43 @@ def test_current(gpu_ctx, which, request):
     else:
         assert rel_err(j, jref) <= FP64_TOL
         assert np.max(np.abs(tot - tref)) <= 1e-12 * np.max(np.abs(tref))
+
+
+@pytest.mark.parametrize("which", ["si8", "si8c"])
+def test_hpsi_spd_projectors(gpu_ctx, which, request):
+    """s + p + d channels (9 KB projectors per atom, as the Al500 form uses): the (-i)^l
+    phases of l = 2 and the 9-column atom blocks."""
+    import math
+    import torch
+    from oracle import hpsi as oh
+    from qball_amd import Hamiltonian
+    sysm = request.getfixturevalue(which)
+    ob, maps, _ = oracle_for(sysm)
+    k = ob.kpgx
+    g = np.sqrt(ob.kpg2)
+    gi = np.where(g > 0, 1.0 / np.where(g > 0, g, 1.0), 0.0)
+    x, y, z = (k[:, d] * gi for d in range(3))
+    vd = 0.7 * g * g * np.exp(-0.45 * g * g)
+    s15 = math.sqrt(15.0 / (4 * math.pi))
+    s5 = math.sqrt(5.0 / (16 * math.pi))
+    dtab = [s15 * x * y * vd, s15 * y * z * vd, s15 * z * x * vd,
+            0.5 * s15 * (x * x - y * y) * vd, s5 * (3 * z * z - 1.0) * vd]
+    twnl = np.concatenate([sysm.twnl, np.stack(dtab)])
+    lproj = np.array(list(sysm.lproj) + [2] * 5, dtype=np.int32)
+    wt = np.array(list(sysm.wt) + [0.8] * 5)
+    sp = oh.Projectors(lproj, wt, twnl, sysm.tau)
+    basis = make_gpu_basis(gpu_ctx, sysm)
+    ham = Hamiltonian(gpu_ctx, basis)
+    vdv = real_to_dev(sysm.v)
+    ham.set_vloc(vdv.data_ptr())
+    ham.set_projectors([sysm.na], [9], lproj, wt, twnl.ravel(), sysm.tau.ravel(), sysm.omega)
+    cd = to_dev(sysm.psi)
+    hd = torch.zeros_like(cd)
+    enl = ham.hpsi(cd.data_ptr(), basis.ngw, sysm.nst, sysm.occ, hd.data_ptr(), basis.ngw,
+                   want_enl=True)
+    gpu_ctx.sync()
+    ref, enl_ref = oh.hpsi(maps, sysm.psi, sysm.v, [sp], sysm.occ, sysm.omega)
+    ham.close()
+    basis.close()
+    assert rel_err(from_dev(hd, basis.ngw, sysm.nst), ref) <= FP64_TOL
+    assert abs(enl - enl_ref) <= 1e-10
