@@ -41,6 +41,8 @@ typedef struct qbh_basis_s* qbh_basis;
 typedef struct qbh_ham_s* qbh_ham;
 typedef struct qbh_td_s* qbh_td;
 typedef struct qbh_batch_s* qbh_batch;
+typedef struct qbh_psda_s* qbh_psda;
+typedef struct qbh_mixer_s* qbh_mixer;
 
 /* ---- context ------------------------------------------------------------------------ */
 /* stream: a hipStream_t to run on, or NULL for a stream owned by the context. */
@@ -186,6 +188,46 @@ int qbh_etrs_step(qbh_ham ham, qbh_basis vbasis, double* psi_dev, long ld, int n
                   const double* occ, double weight, double dt, int order,
                   const double* vion_g_dev, const double* rhops_g_dev, double omega,
                   double* energies, int flags);
+
+/* ---- SCF driver pieces --------------------------------------------------------------------
+ * PSDAWavefunctionStepper (PSDAWavefunctionStepper.cc:38-375) with the Preconditioner of
+ * Preconditioner.cc:47-103 (K_G = 0.5 / max(|k+G|^2/2, ecutprec); qball's ecutprec = 0
+ * means the wavefunction ecut).  update: A = C^T(HC) in the basis metric (Γ: 2 C^T HC -
+ * c0 hc0^T), hpsi_dev <- -K (HC - C A) (the reference also overwrites dwf), then
+ * psi <- psi + descent with the Anderson extrapolation over the previous step (theta,
+ * clamped as the reference does, is returned).  The caller orthonormalises (qbh_gram). */
+int qbh_psda_create(qbh_basis basis, int nst, double ecutprec, qbh_psda* out);
+int qbh_psda_destroy(qbh_psda p);
+int qbh_psda_preprocess(qbh_psda p);  /* clears the extrapolation history (.h:53-58) */
+int qbh_psda_update(qbh_psda p, double* psi_dev, long ld, double* hpsi_dev, long ldh,
+                    const double* occ, double* theta);
+/* sum_n occ_n <c_n| |k+G|^2/2 |c_n> in the basis metric (the ekin sum of
+ * EnergyFunctional::energy, EnergyFunctional.cc:1155-1359). */
+int qbh_kinetic_energy(qbh_basis basis, const double* psi_dev, long ld, int nst,
+                       const double* occ, double* ekin);
+/* AndersonMixer (AndersonMixer.cc) over m doubles with nmax (<= 8) previous pairs:
+ * xbar/fbar = x_k/f_k - sum_i theta_i (x_k - x_{k-i-1}) with theta from the Tikhonov-
+ * regularised least squares (regularisation doubled until sum theta^2 <= 1).  theta
+ * (nullable, host) receives min(updates-1, nmax) values. */
+int qbh_mixer_create(qbh_ctx ctx, long m, int nmax, qbh_mixer* out);
+int qbh_mixer_destroy(qbh_mixer mixer);
+int qbh_mixer_restart(qbh_mixer mixer);
+int qbh_mixer_update(qbh_mixer mixer, const double* x_dev, const double* f_dev, double* xbar_dev,
+                     double* fbar_dev, double* theta);
+/* The BOSampleStepper SCF loop for fixed ions (BOSampleStepper.cc:880-1130) with
+ * wf_dyn = PSDA and Anderson charge mixing: per cycle update_density, mixing of rhog
+ * (Kerker rcut rc_kerker, wls rc1 = 3, coefficient mix_alpha, mix_ndim previous pairs,
+ * BOSampleStepper.cc:394-447, 987-1046), update_rhor, update_vhxc (into ham's v_r), then
+ * nite x (Hψ -> PSDA update -> gram).  The last inner iteration of each cycle uses
+ * flags_last, the others flags_inner (e.g. QBH_ALL|QBH_FP32 then QBH_ALL: one FP64
+ * residual-correction pass per cycle).  history (nullable, host) receives
+ * nitscf*max(nite,1) rows of {etotal, ekin, enl, eps, ehart, exc, theta}; etotal excludes
+ * the ion-ion terms esr - eself, which do not depend on the orbitals. */
+int qbh_scf_run(qbh_ham ham, qbh_basis vbasis, double* psi_dev, long ld, int nst,
+                const double* occ, double omega, const double* vion_g_dev,
+                const double* rhops_g_dev, int nitscf, int nite, double ecutprec,
+                double mix_alpha, int mix_ndim, double rc_kerker, int flags_inner,
+                int flags_last, double* history);
 
 /* ---- launch tuning, autotuner, multi-cell graph (no counterpart in the reference) ------
  * Knobs (every value computes the same Hψ; only speed differs):

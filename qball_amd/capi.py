@@ -55,6 +55,18 @@ SIGNATURES = {
     "qbh_exponential": (_c_int, [_vp, _vp, _c_long, _c_int, _c_double, _c_int, _c_int]),
     "qbh_current": (_c_int, [_vp, _vp, _c_long, _c_int, _dp, _c_double, _c_double, _dp, _c_double,
                              _vp, _dp]),
+    "qbh_psda_create": (_c_int, [_vp, _c_int, _c_double, ctypes.POINTER(_vp)]),
+    "qbh_psda_destroy": (_c_int, [_vp]),
+    "qbh_psda_preprocess": (_c_int, [_vp]),
+    "qbh_psda_update": (_c_int, [_vp, _vp, _c_long, _vp, _c_long, _dp, _dp]),
+    "qbh_kinetic_energy": (_c_int, [_vp, _vp, _c_long, _c_int, _dp, _dp]),
+    "qbh_mixer_create": (_c_int, [_vp, _c_long, _c_int, ctypes.POINTER(_vp)]),
+    "qbh_mixer_destroy": (_c_int, [_vp]),
+    "qbh_mixer_restart": (_c_int, [_vp]),
+    "qbh_mixer_update": (_c_int, [_vp, _vp, _vp, _vp, _vp, _dp]),
+    "qbh_scf_run": (_c_int, [_vp, _vp, _vp, _c_long, _c_int, _dp, _c_double, _vp, _vp, _c_int,
+                             _c_int, _c_double, _c_double, _c_int, _c_double, _c_int, _c_int,
+                             _dp]),
     "qbh_ham_set_tuning": (_c_int, [_vp, ctypes.c_char_p, _c_int]),
     "qbh_ham_get_tuning": (_c_int, [_vp, ctypes.c_char_p, _ip]),
     "qbh_ham_autotune": (_c_int, [_vp, _vp, _c_long, _c_int, _dp, _vp, _c_long, _c_int, _c_int,
@@ -270,6 +282,21 @@ class Hamiltonian:
                                       ctypes.byref(enl) if want_enl else None), "qbh_hpsi")
         return enl.value if want_enl else None
 
+    def scf_run(self, vbasis, psi_ptr, ld, nst, occ, omega, vion_g_ptr, rhops_g_ptr, nitscf,
+                nite, ecutprec, alpha=0.5, ndim=3, rc_kerker=10.0, flags_inner=QBH_ALL,
+                flags_last=QBH_ALL):
+        """BOSampleStepper SCF loop (PSDA + Anderson/Kerker mixing); returns the history
+        (nitscf, max(nite,1), 7): etotal, ekin, enl, eps, ehart, exc, theta."""
+        o = np.ascontiguousarray(occ, dtype=np.float64)
+        nin = max(nite, 1)
+        hist = np.zeros((nitscf, nin, 7))
+        self.ctx.check(lib().qbh_scf_run(self.h, vbasis.h, psi_ptr, ld, nst, o.ctypes.data_as(_dp),
+                                         float(omega), vion_g_ptr, rhops_g_ptr, int(nitscf),
+                                         int(nite), float(ecutprec), float(alpha), int(ndim),
+                                         float(rc_kerker), int(flags_inner), int(flags_last),
+                                         hist.ctypes.data_as(_dp)), "qbh_scf_run")
+        return hist
+
     def set_tuning(self, knob, value):
         self.ctx.check(lib().qbh_ham_set_tuning(self.h, knob.encode(), int(value)),
                        "qbh_ham_set_tuning")
@@ -378,3 +405,66 @@ class Batch:
         if self.h:
             lib().qbh_batch_destroy(self.h)
             self.h = _vp()
+
+
+class PSDA:
+    """PSDA wavefunction stepper (qbh_psda_*) on a basis."""
+
+    def __init__(self, basis, nst, ecutprec):
+        self.ctx = basis.ctx
+        self.h = _vp()
+        self.ctx.check(lib().qbh_psda_create(basis.h, int(nst), float(ecutprec), ctypes.byref(self.h)),
+                       "qbh_psda_create")
+
+    def preprocess(self):
+        self.ctx.check(lib().qbh_psda_preprocess(self.h), "qbh_psda_preprocess")
+
+    def update(self, psi_ptr, ld, hpsi_ptr, ldh, occ):
+        o = np.ascontiguousarray(occ, dtype=np.float64)
+        th = ctypes.c_double(0.0)
+        self.ctx.check(lib().qbh_psda_update(self.h, psi_ptr, ld, hpsi_ptr, ldh,
+                                             o.ctypes.data_as(_dp), ctypes.byref(th)),
+                       "qbh_psda_update")
+        return th.value
+
+    def close(self):
+        if self.h:
+            lib().qbh_psda_destroy(self.h)
+            self.h = _vp()
+
+
+class Mixer:
+    """Anderson mixer (qbh_mixer_*) over m doubles with nmax previous pairs."""
+
+    def __init__(self, ctx, m, nmax):
+        self.ctx = ctx
+        self.h = _vp()
+        self.nmax = nmax
+        self.updates = 0
+        ctx.check(lib().qbh_mixer_create(ctx.h, int(m), int(nmax), ctypes.byref(self.h)),
+                  "qbh_mixer_create")
+
+    def restart(self):
+        self.updates = 0
+        self.ctx.check(lib().qbh_mixer_restart(self.h), "qbh_mixer_restart")
+
+    def update(self, x_ptr, f_ptr, xbar_ptr, fbar_ptr):
+        th = np.zeros(max(self.nmax, 1))
+        self.ctx.check(lib().qbh_mixer_update(self.h, x_ptr, f_ptr, xbar_ptr, fbar_ptr,
+                                              th.ctypes.data_as(_dp)), "qbh_mixer_update")
+        n = min(self.updates, self.nmax)
+        self.updates += 1
+        return th[:n]
+
+    def close(self):
+        if self.h:
+            lib().qbh_mixer_destroy(self.h)
+            self.h = _vp()
+
+
+def kinetic_energy(basis, psi_ptr, ld, nst, occ):
+    o = np.ascontiguousarray(occ, dtype=np.float64)
+    e = ctypes.c_double(0.0)
+    basis.ctx.check(lib().qbh_kinetic_energy(basis.h, psi_ptr, ld, int(nst), o.ctypes.data_as(_dp),
+                                             ctypes.byref(e)), "qbh_kinetic_energy")
+    return e.value

@@ -49,9 +49,10 @@ __global__ void k_sum_band_splits(int M, int N, int nsplit, int cplx, const doub
 }
 
 // S (M x N, ld M) = alpha * A^T B (real view, K rows) or A^H B (complex, K rows)
-static int band_product(Ctx* c, rocblas_handle bh, bool real, int M, int N, long K,
-                        const double* A, long lda, const double* B, long ldb, double alpha,
-                        double* S, double*& scratch, size_t& scratch_bytes) {
+int qbh::band_product(Ctx* c, void* bhv, bool real, int M, int N, long K, const double* A,
+                      long lda, const double* B, long ldb, double alpha, double* S,
+                      double*& scratch, size_t& scratch_bytes) {
+  rocblas_handle bh = static_cast<rocblas_handle>(bhv);
   const long tiles = (long)((M + 127) / 128) * ((N + 127) / 128);
   long ns = std::min<long>((4096 + tiles - 1) / tiles, 32);
   ns = std::max<long>(1, std::min<long>(ns, K / 1024));

@@ -148,6 +148,11 @@ int pipeline_x_load(Ctx* ctx, Basis* b, const PipelineArgs& a, const double* f, 
 int pipeline_x_density(Ctx* ctx, Basis* b, const PipelineArgs& a, const double* facs,
                        double* rho);
 size_t slot_bytes(const Basis* b, int prec);
+// S (M x N, ld M) = alpha * A^T B - a0 b0^T over the real view with K rows (real != 0, the
+// Γ metric when alpha = 2) or A^H B (complex, K rows); K-split dgemm/zgemm (qbh_dense.hip)
+int band_product(Ctx* c, void* blas, bool real, int M, int N, long K, const double* A, long lda,
+                 const double* B, long ldb, double alpha, double* S, double*& scratch,
+                 size_t& scratch_bytes);
 // a two-factor register transform is instantiated for edge length n
 bool register_edge(int n);
 
