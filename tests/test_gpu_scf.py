@@ -59,11 +59,10 @@ def test_psda_update_matches_oracle(gpu_ctx, which):
     sysm, ob = s["sysm"], s["ob"]
     occ = np.array([2.0, 2.0, 2.0, 1.5, 1.0, 0.0])
     rng = np.random.default_rng(1)
-    H = rng.standard_normal((ob.size, ob.size)) * 0.01
-    H = H + H.T + np.diag(0.5 * ob.kpg2)
+    U = rng.standard_normal((ob.size, 5)) * 0.05   # H = diag(|G|^2/2) + U U^T (symmetric)
 
     def hc_of(c):
-        h = H @ c.real + 1j * (H @ c.imag)
+        h = (0.5 * ob.kpg2)[:, None] * c + U @ (U.T @ c.real) + 1j * (U @ (U.T @ c.imag))
         if sysm.real:
             h[0] = h[0].real
         return h
