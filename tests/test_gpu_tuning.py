@@ -85,6 +85,9 @@ def test_autotune_keeps_parity_and_caches(gpu_ctx):
     c["basis"].close()
 
 
+@pytest.mark.skip(reason="qbh_batch_create is switched off: its first GPU run ended in a host "
+                         "segmentation fault inside the call (eager warm-up or stream capture); "
+                         "the cause is not identified")
 @pytest.mark.parametrize("flags", [0x7, 0x17])
 def test_batch_graph_replays_match_eager(flags):
     """Three uneven cells in one graph over two branches; each replay (with eager work and
@@ -134,3 +137,12 @@ def test_batch_graph_replays_match_eager(flags):
         c["ham"].close()
         c["basis"].close()
     ctx.close()
+
+
+def test_batch_create_reports_disabled(gpu_ctx):
+    """The switched-off graph entry point fails with a clear error instead of running."""
+    import ctypes
+    from types import SimpleNamespace
+    from qball_amd import Batch, QbhError
+    with pytest.raises(QbhError, match="disabled"):
+        Batch(gpu_ctx, [(SimpleNamespace(h=ctypes.c_void_p()), 16, 1, 1, 32, 1)])
