@@ -25,6 +25,11 @@ PEAK_FP64_TFLOPS = 78.6
 PEAK_FP32_TFLOPS = 157.3
 
 
+def log(msg):
+    """Progress line on stderr (the JSON result is the only stdout line)."""
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
 def parse():
     p = argparse.ArgumentParser()
     p.add_argument("--gpus", type=int, default=1)
@@ -88,6 +93,7 @@ def main():
     hpsi = torch.empty_like(psi)
     ctx.sync()
     t_setup = time.time() - t_setup
+    log(f"setup done ({t_setup:.1f} s)")
 
     def step(flags):
         ham.hpsi(psi.data_ptr(), ngw, nst, sysm.occ, hpsi.data_ptr(), ngw, flags=flags)
@@ -128,6 +134,7 @@ def main():
     ctx.set_timing(False)
     ktimes = ctx.timing_report()
     dt = (t1 - t0) / args.steps
+    log(f"fp64 timed: {dt * 1e3:.2f} ms/step")
     value = nst / dt
 
     # energy-like check of this application: band sum of <psi|H|psi> (Γ: 2 Re - G=0 term)
@@ -186,6 +193,7 @@ def main():
                for k, (ms, cnt) in ctx.timing_report().items()}
         e32 = band_energy(hpsi)
         rel = float(torch.linalg.vector_norm(hpsi - h64) / torch.linalg.vector_norm(h64))
+        log(f"fp32 timed: {dt32 * 1e3:.2f} ms/step")
         fp32 = {"value": nst / dt32, "ms_per_step": dt32 * 1e3, "speedup_vs_fp64": dt / dt32,
                 "hpsi_rel_err_vs_fp64": rel,
                 "band_energy_diff_per_atom_Ha": abs(e32 - e64) / sysm.na,
@@ -195,11 +203,13 @@ def main():
     batch = None
     if not args.no_batch:
         batch = batch_bench(args)
+        log("batch section done")
 
     # CPU baseline: the oracle (numpy restatement) on a bounded sample, 1 thread
     cpu = None
     if not args.no_cpu_baseline:
         cpu = cpu_baseline(sysm, args.cpu_sample_pairs, psi, ngw, nst)
+        log("cpu baseline done")
 
     out = {
         "metric": "Hψ band-applications/s at Si512 Γ (FP64 plain form)",
