@@ -102,27 +102,18 @@ def main():
         step(QBH_ALL)
     torch.cuda.synchronize()
 
-    # schedule choice (untimed probe): projector GEMMs concurrent with the transforms on
-    # a second stream, or after them on one stream
-    def probe(flags, n=3):
-        torch.cuda.synchronize()
-        t = time.perf_counter()
-        for _ in range(n):
-            step(flags)
-        torch.cuda.synchronize()
-        return (time.perf_counter() - t) / n
+    # launch choices for this shape from the library's autotuner (untimed): transform
+    # variants, x-pass blocking, projector K-split count, GEMM/transform overlap
+    knobs = ("zy_reg", "x_reg", "x_spw", "radix", "nsplit", "overlap")
 
-    def pick_schedule(flags):
-        ctx.set_overlap(True)
-        t_ov = probe(flags)
-        ctx.set_overlap(False)
-        t_seq = probe(flags)
-        ov = t_ov <= t_seq
-        ctx.set_overlap(ov)
-        return {"overlap_ms": t_ov * 1e3, "sequential_ms": t_seq * 1e3,
-                "chosen": "overlap" if ov else "sequential"}
+    def tune(flags):
+        rep = ham.autotune(psi.data_ptr(), ngw, nst, sysm.occ, hpsi.data_ptr(), ngw, flags=flags,
+                           reps=2, force=True)
+        return {"choice": {k: ham.get_tuning(k) for k in knobs},
+                "report": [ln for ln in rep.splitlines() if "->" in ln or ln.startswith("start")]}
 
-    sched64 = pick_schedule(QBH_ALL)
+    sched64 = tune(QBH_ALL)
+    log("fp64 tuned: " + json.dumps(sched64["choice"]))
     ctx.timing_report()                        # discard warm-up totals
     ctx.set_timing(True)
     torch.cuda.synchronize()
@@ -179,7 +170,7 @@ def main():
         h64 = hpsi.clone()
         for _ in range(args.warmup):
             step(QBH_ALL | QBH_FP32)
-        sched32 = pick_schedule(QBH_ALL | QBH_FP32)
+        sched32 = tune(QBH_ALL | QBH_FP32)
         torch.cuda.synchronize()
         ctx.timing_report()
         ctx.set_timing(True)
@@ -197,7 +188,7 @@ def main():
         fp32 = {"value": nst / dt32, "ms_per_step": dt32 * 1e3, "speedup_vs_fp64": dt / dt32,
                 "hpsi_rel_err_vs_fp64": rel,
                 "band_energy_diff_per_atom_Ha": abs(e32 - e64) / sysm.na,
-                "schedule": sched32, "kernels": k32}
+                "tuning": sched32, "kernels": k32}
         del h64
 
     batch = None
@@ -231,7 +222,7 @@ def main():
         "cpu_baseline": cpu,
         "fp32_bulk": fp32,
         "batching": batch,
-        "schedule": sched64,
+        "tuning": sched64,
         "kernels": per_kernel,
         "band_energy_Ha": e64,
         "setup_s": t_setup,

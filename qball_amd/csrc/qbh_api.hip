@@ -328,6 +328,7 @@ int qbh::hpsi_impl(Ham* h, const double* psi, long ld, int nst, const double* oc
   // with the backward/x/forward-y transform passes; the forward z pass, which adds the
   // kinetic and local terms to hpsi, waits for them.
   hipEvent_t join = nullptr;
+  const float* add32 = nullptr;
   if (do_nl) {
     const bool overlap = do_loc && (h->overlap >= 0 ? h->overlap != 0 : c->overlap);
     if (overlap) {
@@ -337,7 +338,9 @@ int qbh::hpsi_impl(Ham* h, const double* psi, long ld, int nst, const double* oc
       c->stream = c->side;
     }
     double dummy = 0.0;
-    const int r = nl_apply(h, psi, ld, nst, occ, hpsi, ldh, fp32 ? 1 : 0, enl ? &dummy : nullptr);
+    // FP32: the forward z pass adds the projector term while it accumulates into hpsi
+    const int r = nl_apply(h, psi, ld, nst, occ, hpsi, ldh, fp32 ? 1 : 0, enl ? &dummy : nullptr,
+                           (fp32 && do_loc) ? &add32 : nullptr);
     if (overlap) {
       hipEventRecord(c->ev_join, c->side);
       c->stream = main;
@@ -358,6 +361,7 @@ int qbh::hpsi_impl(Ham* h, const double* psi, long ld, int nst, const double* oc
       a.band0 = b->g.real ? 2 * s0 : s0;
       a.mode = mode;
       a.wait_z = join;
+      a.add32 = add32;
       if (int r = pipeline_bwd_yz(c, b, a)) return r;
       if (int r = pipeline_x_vmul(c, b, a, h->v)) return r;
       return pipeline_fwd_zy_accumulate(c, b, a, hpsi, ldh, kin ? h->kin : nullptr, 0);

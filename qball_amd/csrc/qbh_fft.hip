@@ -644,7 +644,7 @@ template <typename T, int MODE, int NMAX, int N1, int N2>
 __global__ void __launch_bounds__(NTHR)
 k_zpass_fwd(GridDev g, const cplx<T>* __restrict__ t1, const double* __restrict__ psi, long ld,
             int nst, int band0, double* __restrict__ out, long ldo,
-            const double* __restrict__ kin, int store) {
+            const double* __restrict__ kin, int store, const float* __restrict__ add32) {
   constexpr int EPL = Epl<ZLPW, NMAX>::v;
   extern __shared__ __align__(16) unsigned char smem_raw[];
   const int np2 = g.np2;
@@ -722,6 +722,10 @@ k_zpass_fwd(GridDev g, const cplx<T>* __restrict__ t1, const double* __restrict_
       }
     } else {
       double h0 = o1[2 * ig], h1 = o1[2 * ig + 1];
+      if (add32) {  // FP32 projector term, in the order of a separate add before this pass
+        h0 += double(add32[2 * ((long)n1 * g.ngw + ig)]);
+        h1 += double(add32[2 * ((long)n1 * g.ngw + ig) + 1]);
+      }
       if (kin) {
         const double k = kin[ig];
         h0 += k * q1[2 * ig];
@@ -731,6 +735,10 @@ k_zpass_fwd(GridDev g, const cplx<T>* __restrict__ t1, const double* __restrict_
       o1[2 * ig + 1] = h1 + i1;
       if (has2) {
         double g0 = o2[2 * ig], g1 = o2[2 * ig + 1];
+        if (add32) {
+          g0 += double(add32[2 * ((long)n2 * g.ngw + ig)]);
+          g1 += double(add32[2 * ((long)n2 * g.ngw + ig) + 1]);
+        }
         if (kin) {
           const double k = kin[ig];
           g0 += k * q2[2 * ig];
@@ -893,14 +901,14 @@ static int fwd_zy_t(Ctx* ctx, Basis* b, const PipelineArgs& a, double* out, long
     const size_t lds = lds_z<T>(g);
     if (a.mode == SLOT_PAIR)
       QBH_SIZE_DISPATCH(g.np2, (k_zpass_fwd<T, SLOT_PAIR, NM, F1, F2><<<grid, NTHR, lds, ctx->stream>>>(
-                                   g, t1, a.psi, a.ld, a.nst, a.band0, out, ldo, kin, store)));
+                                   g, t1, a.psi, a.ld, a.nst, a.band0, out, ldo, kin, store, a.add32)));
     else if (a.mode == SLOT_SINGLE_REAL)
       QBH_SIZE_DISPATCH(g.np2,
                         (k_zpass_fwd<T, SLOT_SINGLE_REAL, NM, F1, F2><<<grid, NTHR, lds, ctx->stream>>>(
-                            g, t1, a.psi, a.ld, a.nst, a.band0, out, ldo, kin, store)));
+                            g, t1, a.psi, a.ld, a.nst, a.band0, out, ldo, kin, store, a.add32)));
     else
       QBH_SIZE_DISPATCH(g.np2, (k_zpass_fwd<T, SLOT_COMPLEX, NM, F1, F2><<<grid, NTHR, lds, ctx->stream>>>(
-                                   g, t1, a.psi, a.ld, a.nst, a.band0, out, ldo, kin, store)));
+                                   g, t1, a.psi, a.ld, a.nst, a.band0, out, ldo, kin, store, a.add32)));
   }
   return check_launch(ctx, "zpass_fwd");
 }

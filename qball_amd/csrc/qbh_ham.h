@@ -54,8 +54,10 @@ int hpsi_impl(Ham* h, const double* psi, long ld, int nst, const double* occ, do
 // Taylor exponential of qbh_exponential; krylov (nullable) receives H^k in, k < order
 int expo_impl(Ham* h, const double* in, long ldin, double* outp, long ldout, int nst, double dt,
               int order, int flags, double* krylov);
+// defer_add32 (nullable, FP32 only): instead of adding the FP32 product to hpsi, return
+// it (compact, 2*ngw floats per band) for the forward z pass to add
 int nl_apply(Ham* h, const double* psi, long ld, int nst, const double* occ, double* hpsi,
-             long ldh, int fp32, double* enl);
+             long ldh, int fp32, double* enl, const float** defer_add32 = nullptr);
 double nl_read_enl(Ham* h, hipStream_t s);
 int blas_handle(Ctx* c, void** out);
 void* grow(void*& p, size_t& have, size_t need);

@@ -135,6 +135,7 @@ struct PipelineArgs {
   void* t1;            // workspace
   void* t2;
   hipEvent_t wait_z = nullptr;  // the forward z pass (which writes the output) waits on it
+  const float* add32 = nullptr; // FP32 term (2*ngw floats per band) the forward z pass adds
 };
 
 int pipeline_bwd_yz(Ctx* ctx, Basis* b, const PipelineArgs& a);

@@ -320,7 +320,7 @@ static int fnl_gemm(Ctx* c, rocblas_handle bh, bool cplx, int P, int nst, long K
 }
 
 int nl_apply(Ham* h, const double* psi, long ld, int nst, const double* occ_host, double* hpsi,
-             long ldh, int fp32, double* enl) {
+             long ldh, int fp32, double* enl, const float** defer_add32) {
   Ctx* c = h->ctx;
   if (h->P == 0) {
     if (enl) *enl = 0.0;
@@ -435,7 +435,9 @@ int nl_apply(Ham* h, const double* psi, long ld, int nst, const double* occ_host
                           "enl gemm"))
         return r;
     }
-    {
+    if (defer_add32) {
+      *defer_add32 = t32;  // the forward z pass adds it while accumulating into hpsi
+    } else {
       TimedLaunch tl(c, "add_f32");
       k_add_f32<<<dim3((unsigned)((2 * ngw + 1023) / 1024), (unsigned)nst), 256, 0, c->stream>>>(
           ngw, t32, hpsi, ldh);
