@@ -41,6 +41,11 @@ def parse():
     p.add_argument("--no-fp32", action="store_true")
     p.add_argument("--no-batch", action="store_true", help="skip the 12-cell graph section")
     p.add_argument("--batch-reps", type=int, default=10)
+    p.add_argument("--no-tune", action="store_true",
+                   help="built-in launch choices, sequential schedule (short profiled runs)")
+    p.add_argument("--copy-calibration-gib", type=float, default=0.0,
+                   help="copy this many GiB device-to-device once before the timed region "
+                        "(a plain copy of known size that calibrates HBM byte counters)")
     return p.parse_args()
 
 
@@ -103,16 +108,43 @@ def main():
     torch.cuda.synchronize()
 
     # launch choices for this shape from the library's autotuner (untimed): transform
-    # variants, x-pass blocking, projector K-split count, GEMM/transform overlap
+    # variants, x-pass blocking and the projector K-split count, on the sequential schedule
+    # (GEMMs after the transforms on one stream), so that each kernel's measured duration is
+    # its own; the overlapped schedule is probed and reported beside it
     knobs = ("zy_reg", "x_reg", "x_spw", "radix", "nsplit", "overlap")
 
+    def probe(flags, n=3):
+        torch.cuda.synchronize()
+        t = time.perf_counter()
+        for _ in range(n):
+            step(flags)
+        torch.cuda.synchronize()
+        return (time.perf_counter() - t) / n
+
     def tune(flags):
+        ham.set_tuning("overlap", 0)
         rep = ham.autotune(psi.data_ptr(), ngw, nst, sysm.occ, hpsi.data_ptr(), ngw, flags=flags,
-                           reps=2, force=True)
-        return {"choice": {k: ham.get_tuning(k) for k in knobs},
+                           reps=2, force=True, knobs=knobs[:5])
+        choice = {k: ham.get_tuning(k) for k in knobs}
+        t_seq = probe(flags)
+        ham.set_tuning("overlap", 1)
+        t_ov = probe(flags)
+        ham.set_tuning("overlap", 0)
+        return {"choice": choice, "sequential_ms": t_seq * 1e3, "overlap_probe_ms": t_ov * 1e3,
                 "report": [ln for ln in rep.splitlines() if "->" in ln or ln.startswith("start")]}
 
-    sched64 = tune(QBH_ALL)
+    if args.copy_calibration_gib > 0:
+        n = int(args.copy_calibration_gib * (1 << 30)) // 8
+        a_ = torch.ones(n, dtype=torch.float64, device=dev)
+        b_ = torch.empty_like(a_)
+        b_.copy_(a_)
+        torch.cuda.synchronize()
+        del a_, b_
+    if args.no_tune:
+        ham.set_tuning("overlap", 0)
+        sched64 = {"choice": {k: ham.get_tuning(k) for k in knobs}}
+    else:
+        sched64 = tune(QBH_ALL)
     log("fp64 tuned: " + json.dumps(sched64["choice"]))
     ctx.timing_report()                        # discard warm-up totals
     ctx.set_timing(True)
@@ -170,7 +202,8 @@ def main():
         h64 = hpsi.clone()
         for _ in range(args.warmup):
             step(QBH_ALL | QBH_FP32)
-        sched32 = tune(QBH_ALL | QBH_FP32)
+        sched32 = {"choice": {k: ham.get_tuning(k) for k in knobs}} if args.no_tune \
+            else tune(QBH_ALL | QBH_FP32)
         torch.cuda.synchronize()
         ctx.timing_report()
         ctx.set_timing(True)

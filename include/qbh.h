@@ -241,10 +241,12 @@ int qbh_ham_set_tuning(qbh_ham ham, const char* knob, int value);
 int qbh_ham_get_tuning(qbh_ham ham, const char* knob, int* value);
 /* Times candidate knob values for this Hψ shape (reps applications each, HIP events on the
  * context's stream), keeps the fastest, and caches the choice per shape in the context
- * (force != 0 re-tunes).  hpsi_dev is overwritten.  report (nullable) receives the timings. */
+ * (force != 0 re-tunes).  knob_mask: bit k set = tune knob k in the order listed above
+ * (0 = all); the others keep their current values.  hpsi_dev is overwritten.  report
+ * (nullable) receives the timings. */
 int qbh_ham_autotune(qbh_ham ham, const double* psi_dev, long ld, int nst, const double* occ,
-                     double* hpsi_dev, long ldh, int flags, int reps, int force, char* report,
-                     int cap);
+                     double* hpsi_dev, long ldh, int flags, int reps, int force, int knob_mask,
+                     char* report, int cap);
 /* Hψ of ncells cells (each: hams[i], psi_dev[i] with ld[i] and nst[i] bands, into
  * hpsi_dev[i] with ldh[i]; flags as qbh_hpsi without ACCUMULATE) captured into one
  * hipGraph over nbranch concurrent branches.  The context's stream must be a non-default

@@ -70,7 +70,7 @@ SIGNATURES = {
     "qbh_ham_set_tuning": (_c_int, [_vp, ctypes.c_char_p, _c_int]),
     "qbh_ham_get_tuning": (_c_int, [_vp, ctypes.c_char_p, _ip]),
     "qbh_ham_autotune": (_c_int, [_vp, _vp, _c_long, _c_int, _dp, _vp, _c_long, _c_int, _c_int,
-                                  _c_int, ctypes.c_char_p, _c_int]),
+                                  _c_int, _c_int, ctypes.c_char_p, _c_int]),
     "qbh_batch_create": (_c_int, [_vp, _c_int, ctypes.POINTER(_vp), ctypes.POINTER(_vp),
                                   ctypes.POINTER(_c_long), _ip, ctypes.POINTER(_vp),
                                   ctypes.POINTER(_c_long), _c_int, _c_int, ctypes.POINTER(_vp)]),
@@ -307,13 +307,18 @@ class Hamiltonian:
                        "qbh_ham_get_tuning")
         return v.value
 
-    def autotune(self, psi_ptr, ld, nst, occ, hpsi_ptr, ldh, flags=QBH_ALL, reps=3, force=False):
-        """Times the launch choices for this shape and keeps the fastest; returns the report."""
+    KNOBS = ("zy_reg", "x_reg", "x_spw", "radix", "nsplit", "overlap")
+
+    def autotune(self, psi_ptr, ld, nst, occ, hpsi_ptr, ldh, flags=QBH_ALL, reps=3, force=False,
+                 knobs=None):
+        """Times the launch choices for this shape and keeps the fastest; `knobs` (names)
+        restricts the search (default: all).  Returns the report."""
         o_, op = _da(occ)
+        mask = 0 if knobs is None else sum(1 << self.KNOBS.index(k) for k in knobs)
         buf = ctypes.create_string_buffer(8192)
         self.ctx.check(lib().qbh_ham_autotune(self.h, psi_ptr, ld, nst, op, hpsi_ptr, ldh, flags,
-                                              int(reps), int(bool(force)), buf, len(buf)),
-                       "qbh_ham_autotune")
+                                              int(reps), int(bool(force)), int(mask), buf,
+                                              len(buf)), "qbh_ham_autotune")
         return buf.value.decode()
 
     def exponential(self, psi_ptr, ld, nst, dt, order=4, flags=QBH_ALL):

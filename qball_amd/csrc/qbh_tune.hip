@@ -143,13 +143,14 @@ static std::string shape_key(Ham* h, int nst, int flags) {
 
 extern "C" int qbh_ham_autotune(qbh_ham ham, const double* psi_dev, long ld, int nst,
                                 const double* occ, double* hpsi_dev, long ldh, int flags,
-                                int reps, int force, char* report, int cap) {
+                                int reps, int force, int knob_mask, char* report, int cap) {
   if (!ham || !psi_dev || !hpsi_dev || nst <= 0) return QBH_ERR_ARG;
   Ham* h = &ham->h;
   Ctx* c = h->ctx;
   hipSetDevice(c->device);
   const int fl = (flags & (QBH_ALL | QBH_FP32)) == 0 ? QBH_ALL : (flags & (QBH_ALL | QBH_FP32));
-  const std::string key = shape_key(h, nst, fl);
+  const int mask = knob_mask == 0 ? (1 << NKNOB) - 1 : knob_mask;
+  const std::string key = shape_key(h, nst, fl) + "." + std::to_string(mask);
   std::string rep;
   auto it = c->tune_cache.find(key);
   if (!force && it != c->tune_cache.end()) {
@@ -161,12 +162,19 @@ extern "C" int qbh_ham_autotune(qbh_ham ham, const double* psi_dev, long ld, int
     const bool nl = (fl & QBH_NONLOCAL) && h->P > 0, loc = (fl & QBH_LOCAL) != 0;
     const bool zy = register_edge(g.np1) || register_edge(g.np2);
     std::vector<std::pair<int, std::vector<int>>> plan;
+    // transform knobs first, then the schedule, then the K split (whose best value depends
+    // on whether the GEMMs share the GPU with the transforms)
     if (loc && zy) plan.push_back({0, {1, 0}});
     if (loc && register_edge(g.np0)) plan.push_back({1, {1, 0}});
     if (loc) plan.push_back({3, {0, 1, 2}});
     if (loc) plan.push_back({2, {0, 1, 2, 4, 8, 16}});
-    if (nl) plan.push_back({4, {0, 1, 2, 4, 8, 16, 32}});
     if (nl && loc) plan.push_back({5, {1, 0}});
+    if (nl) plan.push_back({4, {0, 1, 2, 4, 8, 16, 32}});
+    plan.erase(std::remove_if(plan.begin(), plan.end(),
+                              [&](const std::pair<int, std::vector<int>>& kv) {
+                                return !(mask & (1 << kv.first));
+                              }),
+               plan.end());
     float best = 0.0f;
     if (int r = time_hpsi(h, psi_dev, ld, nst, occ, hpsi_dev, ldh, fl, reps, &best)) return r;
     char line[160];
