@@ -108,55 +108,47 @@ def main():
     torch.cuda.synchronize()
 
     # launch choices for this shape from the library's autotuner (untimed): transform
-    # variants, x-pass blocking and the projector K-split count, on the sequential schedule
-    # (GEMMs after the transforms on one stream), so that each kernel's measured duration is
-    # its own; the overlapped schedule is probed and reported beside it
+    # variants, x-pass blocking, GEMM/transform overlap and the projector K-split count
     knobs = ("zy_reg", "x_reg", "x_spw", "radix", "nsplit", "overlap")
+    builtin = {"zy_reg": 1, "x_reg": 1, "x_spw": 0, "radix": 0, "nsplit": 0, "overlap": 0}
 
-    def probe(flags, n=3):
-        torch.cuda.synchronize()
-        t = time.perf_counter()
-        for _ in range(n):
-            step(flags)
-        torch.cuda.synchronize()
-        return (time.perf_counter() - t) / n
+    def apply(choice):
+        for k, v in choice.items():
+            ham.set_tuning(k, v)
 
     def tune(flags):
-        ham.set_tuning("overlap", 0)
         rep = ham.autotune(psi.data_ptr(), ngw, nst, sysm.occ, hpsi.data_ptr(), ngw, flags=flags,
-                           reps=2, force=True, knobs=knobs[:5])
-        choice = {k: ham.get_tuning(k) for k in knobs}
-        t_seq = probe(flags)
-        ham.set_tuning("overlap", 1)
-        t_ov = probe(flags)
-        ham.set_tuning("overlap", 0)
-        return {"choice": choice, "sequential_ms": t_seq * 1e3, "overlap_probe_ms": t_ov * 1e3,
+                           reps=2, force=True)
+        return {"choice": {k: ham.get_tuning(k) for k in knobs},
                 "report": [ln for ln in rep.splitlines() if "->" in ln or ln.startswith("start")]}
 
+    def timed(flags, steps):
+        """(ms per step, per-kernel (total ms, launches)) over `steps` Hψ applications."""
+        ctx.timing_report()                    # discard earlier totals
+        ctx.set_timing(True)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            step(flags)
+        torch.cuda.synchronize()
+        t1 = time.perf_counter()
+        ctx.set_timing(False)
+        return (t1 - t0) / steps, ctx.timing_report()
+
     if args.copy_calibration_gib > 0:
+        # HBM byte-counter calibration: a fill writes and a sum reads exactly this many bytes
         n = int(args.copy_calibration_gib * (1 << 30)) // 8
         a_ = torch.ones(n, dtype=torch.float64, device=dev)
-        b_ = torch.empty_like(a_)
-        b_.copy_(a_)
+        float(a_.sum())
         torch.cuda.synchronize()
-        del a_, b_
+        del a_
     if args.no_tune:
-        ham.set_tuning("overlap", 0)
-        sched64 = {"choice": {k: ham.get_tuning(k) for k in knobs}}
+        apply(builtin)
+        sched64 = {"choice": dict(builtin)}
     else:
         sched64 = tune(QBH_ALL)
-    log("fp64 tuned: " + json.dumps(sched64["choice"]))
-    ctx.timing_report()                        # discard warm-up totals
-    ctx.set_timing(True)
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    for _ in range(args.steps):
-        step(QBH_ALL)
-    torch.cuda.synchronize()
-    t1 = time.perf_counter()
-    ctx.set_timing(False)
-    ktimes = ctx.timing_report()
-    dt = (t1 - t0) / args.steps
+    log("fp64 launch choices: " + json.dumps(sched64["choice"]))
+    dt, ktimes = timed(QBH_ALL, args.steps)
     log(f"fp64 timed: {dt * 1e3:.2f} ms/step")
     value = nst / dt
 
@@ -168,13 +160,25 @@ def main():
         return float((torch.from_numpy(sysm.occ).to(dev) * d).sum())
     e64 = band_energy(hpsi)
 
-    # roofline of the dominant kernel
+    # kernel characterisation: with the GEMMs concurrent to the transforms each kernel's
+    # duration includes the other's share of the GPU, so the per-kernel rooflines come from
+    # the built-in launch choices on the sequential schedule (what `--no-tune` times, and
+    # what the committed rocprof summary of `bench.py --no-tune` shows)
+    char_steps = max(2, args.steps)
+    if args.no_tune:
+        kchar = ktimes
+        char_steps = args.steps
+    else:
+        apply(builtin)
+        _, kchar = timed(QBH_ALL, char_steps)
+        apply(sched64["choice"])
+    traffic = load_traffic()
     nslots_total = (nst + 1) // 2
     chunk_bytes = kernel_bytes(sysm, 1, 16)
     flops_gemm = 2.0 * (2 * ngw) * nst * (4 * sysm.na)  # one GEMM: 2*M*N*K
     per_kernel = {}
-    for name, (ms, cnt) in ktimes.items():
-        calls = cnt / args.steps
+    for name, (ms, cnt) in kchar.items():
+        calls = cnt / char_steps
         avg_ms = ms / cnt
         entry = {"avg_ms": avg_ms, "launches_per_step": calls}
         if name in chunk_bytes:
@@ -186,15 +190,20 @@ def main():
         if name in ("fnl_gemm", "enl_gemm"):
             entry["flops_per_launch"] = flops_gemm
             entry["TFLOPs"] = flops_gemm / (avg_ms * 1e-3) / 1e12
+        if name in traffic:
+            entry["hbm_bytes_per_launch"] = traffic[name]
         per_kernel[name] = entry
     dom = max(per_kernel, key=lambda k: per_kernel[k]["avg_ms"] * per_kernel[k]["launches_per_step"])
     d = per_kernel[dom]
     if "TFLOPs" in d:
         roof = {"kernel": dom, "bound": "mfma", "achieved": d["TFLOPs"], "peak": PEAK_FP64_TFLOPS,
-                "unit": "TFLOP/s", "frac": d["TFLOPs"] / PEAK_FP64_TFLOPS, "traffic": None}
+                "unit": "TFLOP/s", "frac": d["TFLOPs"] / PEAK_FP64_TFLOPS}
     else:
         roof = {"kernel": dom, "bound": "hbm", "achieved": d.get("GBps"), "peak": PEAK_HBM_GBS,
-                "unit": "GB/s", "frac": (d.get("GBps") or 0) / PEAK_HBM_GBS, "traffic": None}
+                "unit": "GB/s", "frac": (d.get("GBps") or 0) / PEAK_HBM_GBS}
+    roof["traffic"] = d.get("hbm_bytes_per_launch")
+    roof["measured_in"] = ("sequential characterisation pass at the built-in launch choices, "
+                           f"{char_steps} steps, HIP events on the library stream")
 
     # FP32-bulk form
     fp32 = None
@@ -202,26 +211,23 @@ def main():
         h64 = hpsi.clone()
         for _ in range(args.warmup):
             step(QBH_ALL | QBH_FP32)
-        sched32 = {"choice": {k: ham.get_tuning(k) for k in knobs}} if args.no_tune \
-            else tune(QBH_ALL | QBH_FP32)
-        torch.cuda.synchronize()
-        ctx.timing_report()
-        ctx.set_timing(True)
-        t0 = time.perf_counter()
-        for _ in range(args.steps):
-            step(QBH_ALL | QBH_FP32)
-        torch.cuda.synchronize()
-        dt32 = (time.perf_counter() - t0) / args.steps
-        ctx.set_timing(False)
-        k32 = {k: {"avg_ms": ms / cnt, "launches_per_step": cnt / args.steps}
-               for k, (ms, cnt) in ctx.timing_report().items()}
+        if args.no_tune:
+            apply(builtin)
+            sched32 = {"choice": dict(builtin)}
+        else:
+            sched32 = tune(QBH_ALL | QBH_FP32)
+        dt32, _ = timed(QBH_ALL | QBH_FP32, args.steps)
+        log(f"fp32 timed: {dt32 * 1e3:.2f} ms/step")
         e32 = band_energy(hpsi)
         rel = float(torch.linalg.vector_norm(hpsi - h64) / torch.linalg.vector_norm(h64))
-        log(f"fp32 timed: {dt32 * 1e3:.2f} ms/step")
+        apply(builtin)
+        _, k32c = timed(QBH_ALL | QBH_FP32, char_steps)
+        k32 = {k: {"avg_ms": ms / cnt, "launches_per_step": cnt / char_steps}
+               for k, (ms, cnt) in k32c.items()}
         fp32 = {"value": nst / dt32, "ms_per_step": dt32 * 1e3, "speedup_vs_fp64": dt / dt32,
                 "hpsi_rel_err_vs_fp64": rel,
                 "band_energy_diff_per_atom_Ha": abs(e32 - e64) / sysm.na,
-                "tuning": sched32, "kernels": k32}
+                "tuning": sched32, "kernels_sequential_builtin": k32}
         del h64
 
     batch = None
@@ -256,7 +262,7 @@ def main():
         "fp32_bulk": fp32,
         "batching": batch,
         "tuning": sched64,
-        "kernels": per_kernel,
+        "kernels_sequential_builtin": per_kernel,
         "band_energy_Ha": e64,
         "setup_s": t_setup,
     }
@@ -264,6 +270,18 @@ def main():
     ham.close()
     basis.close()
     ctx.close()
+
+
+def load_traffic():
+    """HBM bytes per launch of each kernel at Si512 FP64 from rocprofv3 FETCH_SIZE and
+    WRITE_SIZE runs (scaled by the counters' reading for a fill and a sum of known size),
+    committed beside the rocprof summaries; {} when absent."""
+    p = os.path.join(ROOT, "profiles", "traffic_si512_fp64.json")
+    if not os.path.exists(p):
+        return {}
+    with open(p) as f:
+        d = json.load(f)
+    return {k: v["hbm_bytes_per_launch"] for k, v in d.get("kernels", {}).items()}
 
 
 def batch_bench(args):
