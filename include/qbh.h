@@ -259,6 +259,29 @@ int qbh_batch_create(qbh_ctx ctx, int ncells, const qbh_ham* hams, const double*
 int qbh_batch_launch(qbh_batch batch);
 int qbh_batch_destroy(qbh_batch batch);
 
+/* ---- wavefunction files ------------------------------------------------------------------
+ * raw: the per-task block of Wavefunction::write_fast / write_dump (Wavefunction.cc:
+ * 2221-2404) for one process row: ld x nst complex doubles (the c() block, mloc = ld), then
+ * eig[nst] and occ[nst] when both are given (qball writes them with empty states).  append
+ * != 0 appends (several spins / k-points in one file, in the caller's order).  read_raw
+ * starts at byte `offset` and returns the offset after the block.  psi_dev must be complete
+ * (its producing stream synchronised) before a write. */
+int qbh_wf_write_raw(const char* path, int append, const double* psi_dev, long ld, int nst,
+                     const double* eig, const double* occ);
+int qbh_wf_read_raw(const char* path, long offset, double* psi_dev, long ld, int nst,
+                    double* eig, double* occ, long* next_offset);
+/* xml: SlaterDet::write (SlaterDet.cc:2925-3343) for one process row with base64 encoding:
+ * <slater_determinant [spin] kpoint weight size>, the diagonal <density_matrix> (occ), and
+ * per state a <grid_function> of the orbital on the basis grid (backward transform; real
+ * part at Γ, complex values otherwise; native byte order, 72 characters a line).  The basis
+ * must carry the wavefunction basis' own grid (qbh_rods_build's np).  read_xml parses one
+ * such element at or after `offset`, transforms each grid function back to the sphere and
+ * returns the offset after </slater_determinant>; occ (nullable) receives the occupations. */
+int qbh_sd_write_xml(qbh_basis basis, const char* path, int append, const double* psi_dev,
+                     long ld, int nst, const double* occ, double weight, int ispin, int nspin);
+int qbh_sd_read_xml(qbh_basis basis, const char* path, long offset, double* psi_dev, long ld,
+                    int nst, double* occ, long* next_offset);
+
 /* ---- real-time TDDFT over many steps and its adjoint -----------------------------------
  * The EhrenSampleStepper iteration (EhrenSampleStepper.cc:280-1152) for fixed ions, one
  * spin and k-point, complex basis, per step n:

@@ -55,6 +55,13 @@ SIGNATURES = {
     "qbh_exponential": (_c_int, [_vp, _vp, _c_long, _c_int, _c_double, _c_int, _c_int]),
     "qbh_current": (_c_int, [_vp, _vp, _c_long, _c_int, _dp, _c_double, _c_double, _dp, _c_double,
                              _vp, _dp]),
+    "qbh_wf_write_raw": (_c_int, [ctypes.c_char_p, _c_int, _vp, _c_long, _c_int, _dp, _dp]),
+    "qbh_wf_read_raw": (_c_int, [ctypes.c_char_p, _c_long, _vp, _c_long, _c_int, _dp, _dp,
+                                 ctypes.POINTER(_c_long)]),
+    "qbh_sd_write_xml": (_c_int, [_vp, ctypes.c_char_p, _c_int, _vp, _c_long, _c_int, _dp,
+                                  _c_double, _c_int, _c_int]),
+    "qbh_sd_read_xml": (_c_int, [_vp, ctypes.c_char_p, _c_long, _vp, _c_long, _c_int, _dp,
+                                 ctypes.POINTER(_c_long)]),
     "qbh_psda_create": (_c_int, [_vp, _c_int, _c_double, ctypes.POINTER(_vp)]),
     "qbh_psda_destroy": (_c_int, [_vp]),
     "qbh_psda_preprocess": (_c_int, [_vp]),
@@ -473,3 +480,45 @@ def kinetic_energy(basis, psi_ptr, ld, nst, occ):
     basis.ctx.check(lib().qbh_kinetic_energy(basis.h, psi_ptr, ld, int(nst), o.ctypes.data_as(_dp),
                                              ctypes.byref(e)), "qbh_kinetic_energy")
     return e.value
+
+
+def wf_write_raw(path, psi_ptr, ld, nst, eig=None, occ=None, append=False):
+    """Wavefunction::write_fast per-task block (qbh_wf_write_raw)."""
+    e = None if eig is None else np.ascontiguousarray(eig, dtype=np.float64)
+    o = None if occ is None else np.ascontiguousarray(occ, dtype=np.float64)
+    rc = lib().qbh_wf_write_raw(str(path).encode(), int(bool(append)), psi_ptr, int(ld), int(nst),
+                                None if e is None else e.ctypes.data_as(_dp),
+                                None if o is None else o.ctypes.data_as(_dp))
+    if rc != 0:
+        raise QbhError(f"qbh_wf_write_raw failed ({rc})")
+
+
+def wf_read_raw(path, psi_ptr, ld, nst, offset=0, with_eig_occ=False):
+    """Reads one block; returns (next_offset, eig, occ)."""
+    e = np.zeros(nst) if with_eig_occ else None
+    o = np.zeros(nst) if with_eig_occ else None
+    nxt = _c_long(0)
+    rc = lib().qbh_wf_read_raw(str(path).encode(), int(offset), psi_ptr, int(ld), int(nst),
+                               None if e is None else e.ctypes.data_as(_dp),
+                               None if o is None else o.ctypes.data_as(_dp), ctypes.byref(nxt))
+    if rc != 0:
+        raise QbhError(f"qbh_wf_read_raw failed ({rc})")
+    return nxt.value, e, o
+
+
+def sd_write_xml(basis, path, psi_ptr, ld, nst, occ, weight=1.0, ispin=0, nspin=1, append=False):
+    """SlaterDet::write with base64 grid functions (qbh_sd_write_xml)."""
+    o = np.ascontiguousarray(occ, dtype=np.float64)
+    basis.ctx.check(lib().qbh_sd_write_xml(basis.h, str(path).encode(), int(bool(append)), psi_ptr,
+                                           int(ld), int(nst), o.ctypes.data_as(_dp), float(weight),
+                                           int(ispin), int(nspin)), "qbh_sd_write_xml")
+
+
+def sd_read_xml(basis, path, psi_ptr, ld, nst, offset=0):
+    """Reads one <slater_determinant>; returns (next_offset, occupations)."""
+    o = np.zeros(nst)
+    nxt = _c_long(0)
+    basis.ctx.check(lib().qbh_sd_read_xml(basis.h, str(path).encode(), int(offset), psi_ptr,
+                                          int(ld), int(nst), o.ctypes.data_as(_dp),
+                                          ctypes.byref(nxt)), "qbh_sd_read_xml")
+    return nxt.value, o
