@@ -55,7 +55,7 @@ def test_xml_slater_determinant(gpu_ctx, tmp_path, complex_basis):
     text = p.read_text()
     head = ('<slater_determinant kpoint="0 0 0"\n  weight="0.5" size="11">\n'
             '<density_matrix form="diagonal" size="11">\n'
-            + " 2.00000000" * 9 + " 1.50000000\n 0.25000000\n</density_matrix>\n')
+            + " 2.00000000" * 9 + " 1.50000000\n 0.25000000\n</density_matrix>\n")
     assert text.startswith(head)
     assert text.endswith("</slater_determinant>\n")
     blocks = re.findall(r'<grid_function type="(\w+)" nx="(\d+)" ny="(\d+)" nz="(\d+)" '
