@@ -42,17 +42,33 @@ template <typename T> struct Cfg;
 #ifndef QBH_XWPE
 #define QBH_XWPE 0
 #endif
+// y pass: ZB z-planes x LB x-lines per workgroup, LPW lines per wavefront (ZB*LB = 4*LPW).
+// Measured on MI355X at Si512 (140^3): 16 x 1 beats 8 x 2, 4 x 4, 32 x 1 and the 8-line
+// shapes in both precisions (FP64 forward y pass 4.58 -> 4.05 ms, FP32 y passes 5.78 ->
+// 3.63 ms per Hψ): one x-line per workgroup and 256-byte (FP64) column segments.
+#ifndef QBH_YZB64
+#define QBH_YZB64 16
+#endif
+#ifndef QBH_YLB64
+#define QBH_YLB64 1
+#endif
+#ifndef QBH_YZB32
+#define QBH_YZB32 16
+#endif
+#ifndef QBH_YLB32
+#define QBH_YLB32 1
+#endif
 template <> struct Cfg<double> {
   static constexpr int XR = QBH_XR64;  // x pass: rows per wavefront (64 B row segments)
-  static constexpr int LPW = 4;
-  static constexpr int ZB = 8;    // y pass: 8 planes x 16 B = 128 B column segments
-  static constexpr int LB = 2;    // y pass: x-lines per workgroup (ZB*LB = 4*LPW)
+  static constexpr int ZB = QBH_YZB64;  // y pass: z-planes per workgroup (column segments)
+  static constexpr int LB = QBH_YLB64;  // y pass: x-lines per workgroup
+  static constexpr int LPW = ZB * LB / 4;
 };
 template <> struct Cfg<float> {
   static constexpr int XR = QBH_XR32;
-  static constexpr int LPW = 8;
-  static constexpr int ZB = 16;
-  static constexpr int LB = 2;
+  static constexpr int ZB = QBH_YZB32;
+  static constexpr int LB = QBH_YLB32;
+  static constexpr int LPW = ZB * LB / 4;
 };
 
 template <typename T>
