@@ -7,6 +7,11 @@ Si512 at Γ (a = 41.04 bohr, ecut 25 Ry: 72919 plane waves, 1024 bands, 140^3 gr
 One step = one Hψ over all 1024 bands (qbh_hpsi with QBH_ALL), inputs resident in HBM.
 The projector table is generated on the first (warm-up) application and reused, as the
 ions do not move.  Prints one JSON line.
+
+The process does not import torch: it drives libqbh.so through its C ABI with buffers from
+the HIP runtime libqbh.so links (qball_amd.devmem), so the library runs on the system's
+ROCm libraries exactly as a C++ host linking it does (a process that imports torch first
+binds it to the HIP runtime and BLAS libraries bundled with torch instead).
 """
 import argparse
 import json
@@ -43,10 +48,21 @@ def parse():
     p.add_argument("--batch-reps", type=int, default=10)
     p.add_argument("--no-tune", action="store_true",
                    help="built-in launch choices, sequential schedule (short profiled runs)")
-    p.add_argument("--copy-calibration-gib", type=float, default=0.0,
-                   help="copy this many GiB device-to-device once before the timed region "
-                        "(a plain copy of known size that calibrates HBM byte counters)")
+    p.add_argument("--calibrate-gib", type=float, default=0.0,
+                   help="only fill, then sum, a buffer of this many GiB with torch and exit: "
+                        "kernels of known HBM bytes that calibrate the byte counters")
     return p.parse_args()
+
+
+def calibrate(gib):
+    """A fill (writes exactly `gib` GiB) and a sum (reads exactly `gib` GiB) for the
+    rocprofv3 WRITE_SIZE / FETCH_SIZE calibration; a process of its own, with torch."""
+    import torch
+    n = int(gib * (1 << 30)) // 8
+    a = torch.ones(n, dtype=torch.float64, device="cuda:0")
+    s = float(a.sum())
+    torch.cuda.synchronize()
+    print(json.dumps({"calibration_gib": gib, "sum": s}))
 
 
 def kernel_bytes(sysm, nslots, prec_bytes):
@@ -74,50 +90,54 @@ def main():
         print(f"bench.py runs on one GPU only; --gpus {args.gpus} is not supported "
               "(the Hψ path shards by bands; see DESIGN.md)", file=sys.stderr)
         sys.exit(2)
-    import torch
+    if args.calibrate_gib > 0:
+        calibrate(args.calibrate_gib)
+        return
     from qball_amd import Basis, Context, Hamiltonian, QBH_ALL, QBH_FP32
+    from qball_amd.devmem import DeviceArray, synchronize
     from qball_amd.workload import make_system
 
     t_setup = time.time()
     sysm = make_system(ncell=args.ncell, ecut_ry=25.0, seed=0)
-    dev = torch.device("cuda:0")
-    stream = torch.cuda.current_stream()
-    ctx = Context(0, stream.cuda_stream)
+    ctx = Context(0, None)
+    libinfo = ctx.library_info()
+    log(libinfo)
     r = sysm.rods
     basis = Basis(ctx, True, r["rod_h"], r["rod_k"], r["rod_lmin"], r["rod_size"], sysm.np, sysm.b)
     sysm.ncol, sysm.nlx = basis.ncol, basis.nlx
     ham = Hamiltonian(ctx, basis)
-    v = torch.from_numpy(sysm.v.copy()).to(dev)
-    ham.set_vloc(v.data_ptr())
+    v = DeviceArray.from_numpy(sysm.v)
+    ham.set_vloc(v.ptr)
     ham.set_projectors([sysm.na], [4], sysm.lproj, sysm.wt, sysm.twnl.ravel(), sysm.tau.ravel(),
                        sysm.omega)
     ngw, nst = sysm.ngw, sysm.nst
-    psi = torch.from_numpy(np.ascontiguousarray(sysm.psi.T).view(np.float64).ravel().copy()).to(dev)
+    psi = DeviceArray.from_numpy(np.ascontiguousarray(sysm.psi.T).view(np.float64).ravel())
     del sysm.psi
-    basis.gram(psi.data_ptr(), ngw, nst)       # orthonormalise (SlaterDet::gram on the GPU)
-    hpsi = torch.empty_like(psi)
+    basis.gram(psi.ptr, ngw, nst)              # orthonormalise (SlaterDet::gram on the GPU)
+    hpsi = DeviceArray(psi.nbytes)
     ctx.sync()
     t_setup = time.time() - t_setup
     log(f"setup done ({t_setup:.1f} s)")
 
     def step(flags):
-        ham.hpsi(psi.data_ptr(), ngw, nst, sysm.occ, hpsi.data_ptr(), ngw, flags=flags)
+        ham.hpsi(psi.ptr, ngw, nst, sysm.occ, hpsi.ptr, ngw, flags=flags)
 
     for _ in range(args.warmup):
         step(QBH_ALL)
-    torch.cuda.synchronize()
+    synchronize()
 
     # launch choices for this shape from the library's autotuner (untimed): transform
     # variants, x-pass blocking, GEMM/transform overlap and the projector K-split count
-    knobs = ("zy_reg", "x_reg", "x_spw", "radix", "nsplit", "overlap")
-    builtin = {"zy_reg": 1, "x_reg": 1, "x_spw": 0, "radix": 0, "nsplit": 0, "overlap": 0}
+    knobs = ("zy_reg", "x_reg", "x_spw", "radix", "nsplit", "overlap", "fnl_algo", "enl_algo")
+    builtin = {"zy_reg": 1, "x_reg": 1, "x_spw": 0, "radix": 0, "nsplit": 0, "overlap": 0,
+               "fnl_algo": 2, "enl_algo": 2}
 
     def apply(choice):
         for k, v in choice.items():
             ham.set_tuning(k, v)
 
     def tune(flags):
-        rep = ham.autotune(psi.data_ptr(), ngw, nst, sysm.occ, hpsi.data_ptr(), ngw, flags=flags,
+        rep = ham.autotune(psi.ptr, ngw, nst, sysm.occ, hpsi.ptr, ngw, flags=flags,
                            reps=2, force=True)
         return {"choice": {k: ham.get_tuning(k) for k in knobs},
                 "report": [ln for ln in rep.splitlines() if "->" in ln or ln.startswith("start")]}
@@ -126,22 +146,15 @@ def main():
         """(ms per step, per-kernel (total ms, launches)) over `steps` Hψ applications."""
         ctx.timing_report()                    # discard earlier totals
         ctx.set_timing(True)
-        torch.cuda.synchronize()
+        synchronize()
         t0 = time.perf_counter()
         for _ in range(steps):
             step(flags)
-        torch.cuda.synchronize()
+        synchronize()
         t1 = time.perf_counter()
         ctx.set_timing(False)
         return (t1 - t0) / steps, ctx.timing_report()
 
-    if args.copy_calibration_gib > 0:
-        # HBM byte-counter calibration: a fill writes and a sum reads exactly this many bytes
-        n = int(args.copy_calibration_gib * (1 << 30)) // 8
-        a_ = torch.ones(n, dtype=torch.float64, device=dev)
-        float(a_.sum())
-        torch.cuda.synchronize()
-        del a_
     if args.no_tune:
         apply(builtin)
         sched64 = {"choice": dict(builtin)}
@@ -153,12 +166,13 @@ def main():
     value = nst / dt
 
     # energy-like check of this application: band sum of <psi|H|psi> (Γ: 2 Re - G=0 term)
+    psi_h = psi.to_numpy().reshape(nst, 2 * ngw)
+
     def band_energy(h):
-        p = psi.view(nst, ngw, 2)
-        q = h.view(nst, ngw, 2)
-        d = 2.0 * (p * q).sum(dim=(1, 2)) - p[:, 0, 0] * q[:, 0, 0]
-        return float((torch.from_numpy(sysm.occ).to(dev) * d).sum())
-    e64 = band_energy(hpsi)
+        d = 2.0 * np.einsum("ij,ij->i", psi_h, h) - psi_h[:, 0] * h[:, 0]
+        return float(np.dot(sysm.occ, d))
+    h64 = hpsi.to_numpy().reshape(nst, 2 * ngw)
+    e64 = band_energy(h64)
 
     # kernel characterisation: with the GEMMs concurrent to the transforms each kernel's
     # duration includes the other's share of the GPU, so the per-kernel rooflines come from
@@ -208,7 +222,6 @@ def main():
     # FP32-bulk form
     fp32 = None
     if not args.no_fp32:
-        h64 = hpsi.clone()
         for _ in range(args.warmup):
             step(QBH_ALL | QBH_FP32)
         if args.no_tune:
@@ -218,8 +231,10 @@ def main():
             sched32 = tune(QBH_ALL | QBH_FP32)
         dt32, _ = timed(QBH_ALL | QBH_FP32, args.steps)
         log(f"fp32 timed: {dt32 * 1e3:.2f} ms/step")
-        e32 = band_energy(hpsi)
-        rel = float(torch.linalg.vector_norm(hpsi - h64) / torch.linalg.vector_norm(h64))
+        h32 = hpsi.to_numpy().reshape(nst, 2 * ngw)
+        e32 = band_energy(h32)
+        rel = float(np.linalg.norm(h32 - h64) / np.linalg.norm(h64))
+        del h32
         apply(builtin)
         _, k32c = timed(QBH_ALL | QBH_FP32, char_steps)
         k32 = {k: {"avg_ms": ms / cnt, "launches_per_step": cnt / char_steps}
@@ -228,7 +243,7 @@ def main():
                 "hpsi_rel_err_vs_fp64": rel,
                 "band_energy_diff_per_atom_Ha": abs(e32 - e64) / sysm.na,
                 "tuning": sched32, "kernels_sequential_builtin": k32}
-        del h64
+    del h64
 
     batch = None
     if not args.no_batch:
@@ -265,6 +280,7 @@ def main():
         "kernels_sequential_builtin": per_kernel,
         "band_energy_Ha": e64,
         "setup_s": t_setup,
+        "libraries": libinfo,
     }
     print(json.dumps(out))
     ham.close()
@@ -288,8 +304,8 @@ def batch_bench(args):
     """12 uneven cells (cubic boxes 12-28 bohr, 32-48 atoms, 64-320 bands, ecut 30 Ry, Γ),
     FP32-bulk Hψ after FP64 Gram orthonormalisation: one after the other with the built-in
     launch choices, then after the autotuner (the one-graph launch is switched off)."""
-    import torch
     from qball_amd import QBH_ALL, QBH_FP32, Basis, Context, Hamiltonian
+    from qball_amd.devmem import DeviceArray
     from qball_amd.workload import batch_workload, make_batch_cell
     flags = QBH_ALL | QBH_FP32
     ctx = Context(0, None)
@@ -299,13 +315,13 @@ def batch_bench(args):
         r = s.rods
         basis = Basis(ctx, True, r["rod_h"], r["rod_k"], r["rod_lmin"], r["rod_size"], s.np, s.b)
         ham = Hamiltonian(ctx, basis)
-        v = torch.from_numpy(s.v.copy()).cuda()
-        ham.set_vloc(v.data_ptr())
+        v = DeviceArray.from_numpy(s.v)
+        ham.set_vloc(v.ptr)
         ham.set_projectors([s.na], [4], s.lproj, s.wt, s.twnl.ravel(), s.tau.ravel(), s.omega)
-        psi = torch.from_numpy(np.ascontiguousarray(s.psi.T).view(np.float64).ravel().copy()).cuda()
-        torch.cuda.synchronize()
-        basis.gram(psi.data_ptr(), s.ngw, s.nst)
-        out = torch.zeros_like(psi)
+        psi = DeviceArray.from_numpy(np.ascontiguousarray(s.psi.T).view(np.float64).ravel())
+        basis.gram(psi.ptr, s.ngw, s.nst)
+        out = DeviceArray(psi.nbytes)
+        out.zero()
         cells.append(dict(s=s, basis=basis, ham=ham, v=v, psi=psi, out=out))
         del s.psi
     ctx.sync()
@@ -331,7 +347,8 @@ def batch_bench(args):
         c["ham"].autotune(c["psi"].data_ptr(), c["s"].ngw, c["s"].nst, c["s"].occ,
                           c["out"].data_ptr(), c["s"].ngw, flags=flags, reps=3)
         tuned.append({k: c["ham"].get_tuning(k) for k in
-                      ("zy_reg", "x_reg", "x_spw", "radix", "nsplit", "overlap")})
+                      ("zy_reg", "x_reg", "x_spw", "radix", "nsplit", "overlap", "fnl_algo",
+                       "enl_algo")})
     t_tuned = timed(eager, args.batch_reps)
     out = {"cells": len(cells), "bands_total": nb_total, "dtype": "fp32-bulk",
            "value": nb_total / t_tuned, "unit": "band-applications/s",
@@ -357,7 +374,7 @@ def cpu_baseline(sysm, npairs, psi_dev, ngw, nst):
     ob = OB(UnitCell(sysm.a), sysm.ecut)
     maps = FFTMaps(ob, *sysm.np)
     nb = 2 * npairs
-    c = psi_dev[: 2 * ngw * nb].cpu().numpy().view(np.complex128).reshape(nb, ngw).T.copy()
+    c = psi_dev.to_numpy(count=2 * ngw * nb).view(np.complex128).reshape(nb, ngw).T.copy()
     sp = oh.Projectors(sysm.lproj, sysm.wt, sysm.twnl, sysm.tau)
     with threadpool_limits(limits=1):
         t0 = time.perf_counter()

@@ -61,6 +61,10 @@ int qbh_ctx_set_overlap(qbh_ctx ctx, int on);
 /* Copies up to `cap` "name total_ms count" lines (newline separated) into buf and
  * resets the accumulated totals.  Synchronises the stream. */
 int qbh_ctx_timing_report(qbh_ctx ctx, char* buf, int cap);
+/* Versions of the HIP runtime, rocBLAS and hipBLASLt this process has loaded, and which
+ * library the built-in projector GEMM choice uses with them (no counterpart in the
+ * reference; diagnostics). */
+int qbh_ctx_library_info(qbh_ctx ctx, char* buf, int cap);
 
 /* ---- plane-wave basis ------------------------------------------------------------------
  * Replaces the index maps that FourierTransform(basis, np0, np1, np2) builds from a qball
@@ -236,7 +240,12 @@ int qbh_scf_run(qbh_ham ham, qbh_basis vbasis, double* psi_dev, long ld, int nst
  *   "x_spw"   0..64 slots per x-pass workgroup, 0 = occupancy heuristic (basis-wide)
  *   "radix"   0..2 Stockham radix plan variant for the other lengths (basis-wide)
  *   "nsplit"  0..64 K slices of the projector contraction, 0 = heuristic
- *   "overlap" -1/0/1 projector GEMMs concurrent with the transforms, -1 = context default */
+ *   "overlap" -1/0/1 projector GEMMs concurrent with the transforms, -1 = context default
+ *   "fnl_algo", "enl_algo"  -1..16 projector GEMMs F = anl^T psi / hpsi += anl F by
+ *             rocBLAS (0) or by the n-th algorithm hipBLASLt lists for the shape (F over the
+ *             full K; n past the end of the list: the last); -1 = built-in: 2 with
+ *             hipBLASLt 1.2 or newer loaded, else 0 (qbh_ctx_library_info); complex bases,
+ *             for which hipBLASLt lists none, use rocBLAS */
 int qbh_ham_set_tuning(qbh_ham ham, const char* knob, int value);
 int qbh_ham_get_tuning(qbh_ham ham, const char* knob, int* value);
 /* Times candidate knob values for this Hψ shape (reps applications each, HIP events on the

@@ -32,6 +32,7 @@ SIGNATURES = {
     "qbh_ctx_set_timing": (_c_int, [_vp, _c_int]),
     "qbh_ctx_set_overlap": (_c_int, [_vp, _c_int]),
     "qbh_ctx_timing_report": (_c_int, [_vp, ctypes.c_char_p, _c_int]),
+    "qbh_ctx_library_info": (_c_int, [_vp, ctypes.c_char_p, _c_int]),
     "qbh_basis_create": (_c_int, [_vp, _c_int, _c_int, _ip, _ip, _ip, _ip, _c_int, _c_int,
                                   _c_int, _dp, _dp, ctypes.POINTER(_vp)]),
     "qbh_basis_destroy": (_c_int, [_vp]),
@@ -184,6 +185,13 @@ class Context:
             out[name] = (float(ms), int(cnt))
         return out
 
+    def library_info(self):
+        """HIP runtime, rocBLAS and hipBLASLt versions loaded in this process, and the
+        library the built-in projector GEMM choice uses with them."""
+        buf = ctypes.create_string_buffer(512)
+        self.check(lib().qbh_ctx_library_info(self.h, buf, len(buf)), "library_info")
+        return buf.value.decode()
+
     def close(self):
         if self.h:
             lib().qbh_ctx_destroy(self.h)
@@ -314,7 +322,7 @@ class Hamiltonian:
                        "qbh_ham_get_tuning")
         return v.value
 
-    KNOBS = ("zy_reg", "x_reg", "x_spw", "radix", "nsplit", "overlap")
+    KNOBS = ("zy_reg", "x_reg", "x_spw", "radix", "nsplit", "overlap", "fnl_algo", "enl_algo")
 
     def autotune(self, psi_ptr, ld, nst, occ, hpsi_ptr, ldh, flags=QBH_ALL, reps=3, force=False,
                  knobs=None):

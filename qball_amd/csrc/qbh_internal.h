@@ -94,6 +94,7 @@ struct Ctx {
   hipStream_t stream = nullptr;
   bool own_stream = false;
   void* blas = nullptr;  // rocblas_handle
+  void* lt = nullptr;    // hipBLASLt handle, workspace and per-shape plans (qbh_nl.hip)
   std::string last_error;
   KernelTimer timer;
   // second stream (default priority) for the projector GEMMs, forked from and joined to

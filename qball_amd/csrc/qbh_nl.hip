@@ -8,13 +8,18 @@
 //
 // The projector table anl is generated once per geometry / vector potential and kept in
 // HBM (2*ngw*P doubles; 2.4 GB for 512 Si atoms), so an Hψ call streams it twice
-// through the two MFMA GEMMs (rocBLAS dgemm/zgemm/sgemm/cgemm).
+// through the two MFMA GEMMs: rocBLAS (dgemm/zgemm/sgemm/cgemm, F split over K slices) or,
+// for the real Γ view, hipBLASLt with a tuned algorithm over the full K.
 #include <hip/hip_runtime.h>
+#include <hipblaslt/hipblaslt.h>
 #include <rocblas/rocblas.h>
 
 #include <algorithm>
+#include <array>
 #include <cstdio>
+#include <map>
 #include <string>
+#include <vector>
 
 #include "../../include/qbh.h"
 #include "qbh_ham.h"
@@ -45,9 +50,130 @@ int blas_handle(Ctx* c, void** out) {
   return 0;
 }
 
+// hipBLASLt state of a context: the handle, one workspace, and per GEMM shape the
+// descriptors and the algorithms the heuristic lists for it (in the order listed, so a
+// tuned index names the same algorithm for the same shape and library)
+struct LtPlan {
+  hipblasLtMatmulDesc_t op = nullptr;
+  hipblasLtMatrixLayout_t a = nullptr, b = nullptr, c = nullptr;
+  std::vector<hipblasLtMatmulHeuristicResult_t> cand;
+};
+using LtKey = std::array<long, 8>;  // fp64, transA, m, n, k, lda, ldb, ldc
+struct LtState {
+  hipblasLtHandle_t h = nullptr;
+  bool failed = false;  // handle creation failed: every shape uses rocBLAS
+  int version = 0;      // hipblasLtGetVersion of the library loaded in this process
+  void* ws = nullptr;
+  size_t ws_bytes = 0;
+  std::map<LtKey, LtPlan> plans;
+};
+constexpr size_t kLtWorkspaceCap = size_t(256) << 20;
+constexpr int kLtCandidates = 16;
+
 void blas_destroy(Ctx* c) {
   if (c->blas) rocblas_destroy_handle(static_cast<rocblas_handle>(c->blas));
   c->blas = nullptr;
+  if (c->lt) {
+    LtState* lt = static_cast<LtState*>(c->lt);
+    for (auto& kv : lt->plans) {
+      if (kv.second.a) hipblasLtMatrixLayoutDestroy(kv.second.a);
+      if (kv.second.b) hipblasLtMatrixLayoutDestroy(kv.second.b);
+      if (kv.second.c) hipblasLtMatrixLayoutDestroy(kv.second.c);
+      if (kv.second.op) hipblasLtMatmulDescDestroy(kv.second.op);
+    }
+    if (lt->ws) hipFree(lt->ws);
+    if (lt->h) hipblasLtDestroy(lt->h);
+    delete lt;
+    c->lt = nullptr;
+  }
+}
+
+static bool lt_ready(Ctx* c) {
+  if (!c->lt) c->lt = new LtState();
+  LtState* lt = static_cast<LtState*>(c->lt);
+  if (lt->failed) return false;
+  if (!lt->h) {
+    if (hipblasLtCreate(&lt->h) != HIPBLAS_STATUS_SUCCESS) {
+      lt->h = nullptr;
+      lt->failed = true;
+      return false;
+    }
+    if (hipblasLtGetVersion(lt->h, &lt->version) != HIPBLAS_STATUS_SUCCESS) lt->version = 0;
+  }
+  return true;
+}
+
+// Built-in projector GEMM choice: hipBLASLt's second listed algorithm where the loaded
+// hipBLASLt is 1.2 or newer (ROCm 7.2), rocBLAS otherwise.  On MI355X with hipBLASLt 1.2
+// (512-atom Si, FP64 and FP32) every one of the 16 listed algorithms beat rocBLAS on both
+// products and the second was within 1% of the fastest; an older hipBLASLt in the same
+// process (as bundled with a PyTorch build for ROCm 7.0) listed 16x16-tile FP64 kernels
+// about 5x slower than rocBLAS for the same shapes.
+int nl_builtin_algo(Ctx* c) {
+  if (!lt_ready(c)) return 0;
+  return static_cast<LtState*>(c->lt)->version >= 100200 ? 2 : 0;
+}
+
+Ctx* ctx_of(qbh_ctx c);
+
+// D (m x n, ld ldc) = op(A) B + beta D over real FP64 or FP32 column-major operands, op(A) =
+// A^T (A k x m, ld lda) or A (m x k); the plan is built on first use of the shape.  Returns
+// nullptr when hipBLASLt lists no algorithm for it.
+static LtPlan* lt_plan(Ctx* c, bool f64, bool ta, long m, long n, long k, long lda, long ldb,
+                       long ldc) {
+  if (!c->lt) c->lt = new LtState();
+  LtState* lt = static_cast<LtState*>(c->lt);
+  if (lt->failed) return nullptr;
+  if (!lt_ready(c)) return nullptr;
+  const LtKey key = {f64 ? 1L : 0L, ta ? 1L : 0L, m, n, k, lda, ldb, ldc};
+  auto it = lt->plans.find(key);
+  if (it != lt->plans.end()) return it->second.cand.empty() ? nullptr : &it->second;
+  LtPlan& p = lt->plans[key];
+  const hipDataType dt = f64 ? HIP_R_64F : HIP_R_32F;
+  const hipblasOperation_t opa = ta ? HIPBLAS_OP_T : HIPBLAS_OP_N, opb = HIPBLAS_OP_N;
+  bool ok = hipblasLtMatmulDescCreate(&p.op, f64 ? HIPBLAS_COMPUTE_64F : HIPBLAS_COMPUTE_32F,
+                                      dt) == HIPBLAS_STATUS_SUCCESS;
+  ok = ok && hipblasLtMatmulDescSetAttribute(p.op, HIPBLASLT_MATMUL_DESC_TRANSA, &opa,
+                                             sizeof(opa)) == HIPBLAS_STATUS_SUCCESS;
+  ok = ok && hipblasLtMatmulDescSetAttribute(p.op, HIPBLASLT_MATMUL_DESC_TRANSB, &opb,
+                                             sizeof(opb)) == HIPBLAS_STATUS_SUCCESS;
+  ok = ok && hipblasLtMatrixLayoutCreate(&p.a, dt, ta ? k : m, ta ? m : k, lda) ==
+                 HIPBLAS_STATUS_SUCCESS;
+  ok = ok && hipblasLtMatrixLayoutCreate(&p.b, dt, k, n, ldb) == HIPBLAS_STATUS_SUCCESS;
+  ok = ok && hipblasLtMatrixLayoutCreate(&p.c, dt, m, n, ldc) == HIPBLAS_STATUS_SUCCESS;
+  if (ok) {
+    hipblasLtMatmulPreference_t pref;
+    if (hipblasLtMatmulPreferenceCreate(&pref) == HIPBLAS_STATUS_SUCCESS) {
+      uint64_t cap = kLtWorkspaceCap;
+      hipblasLtMatmulHeuristicResult_t res[kLtCandidates];
+      int nres = 0;
+      if (hipblasLtMatmulPreferenceSetAttribute(pref, HIPBLASLT_MATMUL_PREF_MAX_WORKSPACE_BYTES,
+                                                &cap, sizeof(cap)) == HIPBLAS_STATUS_SUCCESS &&
+          hipblasLtMatmulAlgoGetHeuristic(lt->h, p.op, p.a, p.b, p.c, p.c, pref, kLtCandidates,
+                                          res, &nres) == HIPBLAS_STATUS_SUCCESS)
+        for (int i = 0; i < std::min(nres, kLtCandidates); ++i)
+          if (res[i].workspaceSize <= kLtWorkspaceCap) p.cand.push_back(res[i]);
+      hipblasLtMatmulPreferenceDestroy(pref);
+    }
+  }
+  return p.cand.empty() ? nullptr : &p;
+}
+
+// runs candidate `algo` of plan p on the context's stream (D = C in place)
+template <typename T>
+static int lt_gemm(Ctx* c, LtPlan* p, int algo, const T* A, const T* B, T* C, T beta) {
+  LtState* lt = static_cast<LtState*>(c->lt);
+  const hipblasLtMatmulHeuristicResult_t& r = p->cand[algo];
+  if (r.workspaceSize > lt->ws_bytes && !grow(lt->ws, lt->ws_bytes, r.workspaceSize)) {
+    c->last_error = "hipBLASLt workspace allocation failed";
+    return QBH_ERR_ALLOC;
+  }
+  const T one = T(1);
+  const hipblasStatus_t st = hipblasLtMatmul(lt->h, p->op, &one, A, p->a, B, p->b, &beta, C, p->c,
+                                             C, p->c, &r.algo, lt->ws, lt->ws_bytes, c->stream);
+  if (st == HIPBLAS_STATUS_SUCCESS) return 0;
+  c->last_error = "hipBLASLt matmul failed (status " + std::to_string((int)st) + ")";
+  return QBH_ERR_BLAS;
 }
 
 // ------------------------------------------------------------------------------------
@@ -319,6 +445,21 @@ static int fnl_gemm(Ctx* c, rocblas_handle bh, bool cplx, int P, int nst, long K
   return blas_ok(c, st, "fnl gemm");
 }
 
+// hipBLASLt plans of the two projector GEMMs of an Hψ call (real Γ view only): fnl is
+// F (P x nst) = anl^T psi over K = 2 ngw rows, enl is hpsi (K x nst) += anl F (FP64, into
+// hpsi) or tmp = anl F (FP32, into the compact product buffer).  nullptr: use rocBLAS.
+static LtPlan* nl_plan(Ham* h, int which, int nst, long ld, long ldh, bool fp32) {
+  if (!h->b->g.real || h->P == 0) return nullptr;
+  const long K = 2L * h->b->g.ngw, P = h->P;
+  if (which == 0) return lt_plan(h->ctx, !fp32, true, P, nst, K, K, fp32 ? K : 2 * ld, P);
+  return lt_plan(h->ctx, !fp32, false, K, nst, P, K, P, fp32 ? K : 2 * ldh);
+}
+
+int nl_lt_candidates(Ham* h, int which, int nst, long ld, long ldh, int fp32) {
+  LtPlan* p = nl_plan(h, which, nst, ld, ldh, fp32 != 0);
+  return p ? (int)p->cand.size() : 0;
+}
+
 int nl_apply(Ham* h, const double* psi, long ld, int nst, const double* occ_host, double* hpsi,
              long ldh, int fp32, double* enl, const float** defer_add32) {
   Ctx* c = h->ctx;
@@ -363,9 +504,18 @@ int nl_apply(Ham* h, const double* psi, long ld, int nst, const double* occ_host
 
   const long Kelem = real ? 2L * ngw : (long)ngw;  // contraction length in elements
   const int e = cplx ? 2 : 1;
+  // hipBLASLt algorithms chosen for this shape (nullptr: rocBLAS)
+  const int fsel = h->fnl_algo >= 0 ? h->fnl_algo : nl_builtin_algo(c);
+  const int esel = h->enl_algo >= 0 ? h->enl_algo : nl_builtin_algo(c);
+  LtPlan* fplan = fsel > 0 ? nl_plan(h, 0, nst, ld, ldh, fp32 != 0) : nullptr;
+  LtPlan* eplan = esel > 0 ? nl_plan(h, 1, nst, ld, ldh, fp32 != 0) : nullptr;
+  const int falgo = fplan ? std::min<int>(fsel, (int)fplan->cand.size()) - 1 : -1;
+  const int ealgo = eplan ? std::min<int>(esel, (int)eplan->cand.size()) - 1 : -1;
   // K slices of kc elements; the slice count actually written (ceil(K / kc)) can be below
-  // the requested split count, and the finish kernel sums exactly that many partials
-  const int want = h->nsplit > 0 ? h->nsplit : fnl_splits(P, nst, Kelem);
+  // the requested split count, and the finish kernel sums exactly that many partials.
+  // hipBLASLt contracts the full K in one product.
+  const int want =
+      fplan ? 1 : (h->nsplit > 0 ? h->nsplit : fnl_splits(P, nst, Kelem));
   const long kc = (Kelem + want - 1) / want;
   const int nsplit = (int)((Kelem + kc - 1) / kc);
   const long pstride = (long)P * nst * e;
@@ -376,8 +526,9 @@ int nl_apply(Ham* h, const double* psi, long ld, int nst, const double* occ_host
     if (!F) return QBH_ERR_ALLOC;
     {
       TimedLaunch tl(c, "fnl_gemm");
-      if (int r = fnl_gemm<double>(c, bh, cplx, P, nst, Kelem, anl, Kelem, psi,
-                                   real ? 2 * ld : ld, F, kc))
+      if (int r = fplan ? lt_gemm<double>(c, fplan, falgo, anl, psi, F, 0.0)
+                        : fnl_gemm<double>(c, bh, cplx, P, nst, Kelem, anl, Kelem, psi,
+                                           real ? 2 * ld : ld, F, kc))
         return r;
     }
     {
@@ -391,12 +542,15 @@ int nl_apply(Ham* h, const double* psi, long ld, int nst, const double* occ_host
                                                                h->col_fac, occ, h->red);
     }
     TimedLaunch tl(c, "enl_gemm");
-    if (int r = blas_ok(c,
-                        gemm_sb(bh, rocblas_operation_none, rocblas_operation_none,
-                                (int)Kelem, nst, P, anl, Kelem, 0, F, P, 0, hpsi,
-                                real ? 2 * ldh : ldh, 0, 1, cplx, 1.0),
-                        "enl gemm"))
+    if (eplan) {
+      if (int r = lt_gemm<double>(c, eplan, ealgo, anl, F, hpsi, 1.0)) return r;
+    } else if (int r = blas_ok(c,
+                               gemm_sb(bh, rocblas_operation_none, rocblas_operation_none,
+                                       (int)Kelem, nst, P, anl, Kelem, 0, F, P, 0, hpsi,
+                                       real ? 2 * ldh : ldh, 0, 1, cplx, 1.0),
+                               "enl gemm")) {
       return r;
+    }
   } else {
     float* F = static_cast<float*>(grow(h->f32, h->f32_bytes, fsz * sizeof(float)));
     const size_t psz = 2 * (size_t)ngw * nst;
@@ -414,7 +568,9 @@ int nl_apply(Ham* h, const double* psi, long ld, int nst, const double* occ_host
     }
     {
       TimedLaunch tl(c, "fnl_gemm");
-      if (int r = fnl_gemm<float>(c, bh, cplx, P, nst, Kelem, anl, Kelem, p32, Kelem, F, kc))
+      if (int r = fplan ? lt_gemm<float>(c, fplan, falgo, anl, p32, F, 0.0f)
+                        : fnl_gemm<float>(c, bh, cplx, P, nst, Kelem, anl, Kelem, p32, Kelem,
+                                          F, kc))
         return r;
     }
     {
@@ -429,11 +585,15 @@ int nl_apply(Ham* h, const double* psi, long ld, int nst, const double* occ_host
     }
     {
       TimedLaunch tl(c, "enl_gemm");
-      if (int r = blas_ok(c,
-                          gemm_sb(bh, rocblas_operation_none, rocblas_operation_none, (int)Kelem,
-                                  nst, P, anl, Kelem, 0, F, P, 0, t32, Kelem, 0, 1, cplx, 0.0),
-                          "enl gemm"))
+      if (eplan) {
+        if (int r = lt_gemm<float>(c, eplan, ealgo, anl, F, t32, 0.0f)) return r;
+      } else if (int r = blas_ok(c,
+                                 gemm_sb(bh, rocblas_operation_none, rocblas_operation_none,
+                                         (int)Kelem, nst, P, anl, Kelem, 0, F, P, 0, t32, Kelem,
+                                         0, 1, cplx, 0.0),
+                                 "enl gemm")) {
         return r;
+      }
     }
     if (defer_add32) {
       *defer_add32 = t32;  // the forward z pass adds it while accumulating into hpsi
@@ -455,6 +615,25 @@ int nl_apply(Ham* h, const double* psi, long ld, int nst, const double* occ_host
   }
   return 0;
 }
+
+}  // namespace qbh
+
+extern "C" int qbh_ctx_library_info(qbh_ctx ctx, char* buf, int cap) {
+  using namespace qbh;
+  if (!ctx || !buf || cap <= 0) return QBH_ERR_ARG;
+  Ctx* c = ctx_of(ctx);
+  hipSetDevice(c->device);
+  char rb[128] = {0};
+  rocblas_get_version_string(rb, sizeof rb);
+  int hv = 0;
+  hipRuntimeGetVersion(&hv);
+  const int lv = lt_ready(c) ? static_cast<LtState*>(c->lt)->version : 0;
+  std::snprintf(buf, (size_t)cap, "hip %d rocblas %s hipblaslt %d projector-gemm-builtin %s", hv,
+                rb, lv, nl_builtin_algo(c) ? "hipblaslt" : "rocblas");
+  return QBH_OK;
+}
+
+namespace qbh {
 
 double nl_read_enl(Ham* h, hipStream_t s) {
   double v = 0.0;

@@ -5,7 +5,9 @@
 //          pass), x_spw (slots per x-pass workgroup, 0 = occupancy heuristic), radix
 //          (Stockham radix plan variant for lengths without a register transform);
 //   ham:   nsplit (K slices of the projector contraction, 0 = heuristic), overlap (projector
-//          GEMMs on a side stream concurrent with the transforms, -1 = context default).
+//          GEMMs on a side stream concurrent with the transforms, -1 = context default),
+//          fnl_algo / enl_algo (projector GEMMs by rocBLAS (0) or by the n-th algorithm
+//          hipBLASLt lists for the shape (n >= 1)).
 // qbh_ham_autotune times candidate values with HIP events on the context's stream, one
 // knob at a time (coordinate descent from the current choice), and caches the result per
 // problem shape in the context.
@@ -40,8 +42,10 @@ static int ufail(Ctx* c, int code, const std::string& m) {
   return code;
 }
 
-static const char* const KNOBS[] = {"zy_reg", "x_reg", "x_spw", "radix", "nsplit", "overlap"};
-constexpr int NKNOB = 6;
+static const char* const KNOBS[] = {"zy_reg", "x_reg",   "x_spw",    "radix",
+                                    "nsplit", "overlap", "fnl_algo", "enl_algo"};
+constexpr int NKNOB = 8;
+constexpr int kMaxAlgo = 16;  // hipBLASLt algorithms listed per GEMM shape
 
 static int knob_index(const char* name) {
   for (int i = 0; i < NKNOB; ++i)
@@ -57,7 +61,9 @@ static int get_knob(Ham* h, int k) {
     case 2: return b->tune.x_spw;
     case 3: return b->tune.radix;
     case 4: return h->nsplit;
-    default: return h->overlap;
+    case 5: return h->overlap;
+    case 6: return h->fnl_algo;
+    default: return h->enl_algo;
   }
 }
 
@@ -84,9 +90,13 @@ static int set_knob(Ham* h, int k, int v) {
       if (v < 0 || v > 64) return QBH_ERR_ARG;
       h->nsplit = v;
       return QBH_OK;
-    default:
+    case 5:
       if (v < -1 || v > 1) return QBH_ERR_ARG;
       h->overlap = v;
+      return QBH_OK;
+    default:
+      if (v < -1 || v > kMaxAlgo) return QBH_ERR_ARG;
+      (k == 6 ? h->fnl_algo : h->enl_algo) = v;
       return QBH_OK;
   }
 }
@@ -170,6 +180,18 @@ extern "C" int qbh_ham_autotune(qbh_ham ham, const double* psi_dev, long ld, int
     if (loc) plan.push_back({2, {0, 1, 2, 4, 8, 16}});
     if (nl && loc) plan.push_back({5, {1, 0}});
     if (nl) plan.push_back({4, {0, 1, 2, 4, 8, 16, 32}});
+    // then the GEMM library and algorithm, against the tuned rocBLAS schedule
+    for (int which = 0; nl && which < 2; ++which) {
+      const int na = nl_lt_candidates(h, which, nst, ld, ldh, (fl & QBH_FP32) ? 1 : 0);
+      std::vector<int> vals;
+      for (int v = 0; v <= std::min(na, kMaxAlgo); ++v) vals.push_back(v);
+      if (na > 0) {
+        // start from the value the built-in choice stands for, so it is not timed twice
+        int& knob = which ? h->enl_algo : h->fnl_algo;
+        if (knob < 0) knob = nl_builtin_algo(c);
+        plan.push_back({6 + which, vals});
+      }
+    }
     plan.erase(std::remove_if(plan.begin(), plan.end(),
                               [&](const std::pair<int, std::vector<int>>& kv) {
                                 return !(mask & (1 << kv.first));

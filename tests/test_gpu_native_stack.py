@@ -1,0 +1,36 @@
+"""Parity of Hψ on the system ROCm libraries (HIP runtime, rocBLAS, hipBLASLt as a C++ host
+links them), checked in a child process that never imports torch: every projector GEMM
+algorithm gives the oracle's Hψ, and the built-in choice is hipBLASLt there."""
+import json
+import os
+import subprocess
+import sys
+
+import pytest
+
+pytestmark = pytest.mark.gpu
+HERE = os.path.dirname(os.path.abspath(__file__))
+
+
+def test_hpsi_parity_on_system_rocm_libraries():
+    p = subprocess.run([sys.executable, "-u", os.path.join(HERE, "native_stack_check.py")],
+                       capture_output=True, text=True, timeout=240)
+    assert p.returncode == 0, p.stderr[-3000:]
+    out = json.loads(p.stdout.strip().splitlines()[-1])
+    print(out["libraries"])
+    assert "hipblaslt" in out["libraries"]
+    # this image's ROCm (hipBLASLt 1.2): the built-in projector GEMMs are hipBLASLt's
+    assert out["libraries"].endswith("projector-gemm-builtin hipblaslt"), out["libraries"]
+    assert len(out["cases"]) == 2 * 2 * (1 + 2 * 17)
+    for c in out["cases"]:
+        tol = 1e-12 if c["flags"] == 0x7 else 2e-6
+        assert c["err"] <= tol, c
+
+
+def test_library_info_in_a_torch_process(gpu_ctx):
+    """In a process that imported torch first the library runs on torch's bundled HIP/BLAS
+    libraries; the report names their versions and the built-in GEMM library for them."""
+    info = gpu_ctx.library_info()
+    print(info)
+    assert info.startswith("hip ") and " rocblas " in info and " hipblaslt " in info
+    assert info.split()[-1] in ("hipblaslt", "rocblas")

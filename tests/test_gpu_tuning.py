@@ -54,15 +54,38 @@ def test_every_knob_value_keeps_parity(gpu_ctx, which, grid):
     c = _cell(gpu_ctx, which, grid)
     ref = _ref(c)
     ham = c["ham"]
+    # the K split applies to the rocBLAS product (fnl_algo 0)
     cases = [("radix", 1), ("radix", 2), ("radix", 0), ("x_spw", 1), ("x_spw", 3), ("x_spw", 16),
              ("x_spw", 0), ("zy_reg", 0), ("x_reg", 0), ("zy_reg", 1), ("x_reg", 1),
+             ("fnl_algo", 0), ("enl_algo", 0),
              ("nsplit", 1), ("nsplit", 3), ("nsplit", 7), ("nsplit", 32), ("nsplit", 0),
-             ("overlap", 0), ("overlap", 1), ("overlap", -1)]
+             ("overlap", 0), ("overlap", 1), ("overlap", -1),
+             ("fnl_algo", 1), ("enl_algo", 1), ("fnl_algo", 16), ("enl_algo", 16),
+             ("fnl_algo", 2), ("enl_algo", 2)]
     for knob, val in cases:
         ham.set_tuning(knob, val)
         assert ham.get_tuning(knob) == val
         got = _run(gpu_ctx, c)
         assert rel_err(got, ref) <= TOL, (knob, val)
+    ham.close()
+    c["basis"].close()
+
+
+@pytest.mark.parametrize("flags,tol", [(0x7, TOL), (0x17, 2e-6)])
+def test_every_gemm_algorithm_keeps_parity(gpu_ctx, flags, tol):
+    """Each projector GEMM algorithm hipBLASLt lists for the shape (F over the full K)
+    gives the oracle's Hψ, in FP64 and in the FP32 form."""
+    c = _cell(gpu_ctx, "si8", None, seed=3)
+    ref = _ref(c)
+    ham = c["ham"]
+    for knob in ("fnl_algo", "enl_algo"):
+        for v in range(1, 17):
+            ham.set_tuning(knob, v)
+            assert rel_err(_run(gpu_ctx, c, flags), ref) <= tol, (knob, v)
+        ham.set_tuning(knob, 0)
+    ham.set_tuning("fnl_algo", 3)
+    ham.set_tuning("enl_algo", 5)
+    assert rel_err(_run(gpu_ctx, c, flags), ref) <= tol
     ham.close()
     c["basis"].close()
 
@@ -77,6 +100,8 @@ def test_autotune_keeps_parity_and_caches(gpu_ctx):
                             c["basis"].ngw, reps=2)
     print(rep)
     assert "->" in rep
+    # a Γ basis: hipBLASLt lists algorithms for both projector products, and they are tuned
+    assert "fnl_algo ->" in rep and "enl_algo ->" in rep
     assert rel_err(_run(gpu_ctx, c), _ref(c)) <= TOL
     rep2 = c["ham"].autotune(cd.data_ptr(), c["basis"].ngw, s.nst, s.occ, hd.data_ptr(),
                              c["basis"].ngw, reps=2)
