@@ -112,11 +112,25 @@ k_zpass_bwd(GridDev g, const double* __restrict__ psi, long ld, int nst, int ban
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
   const int grp = blockIdx.x;
   const int slot = blockIdx.y;
+  // this wavefront's lines and the rod of the line its 16-lane group gathers, read before
+  // the workgroup barrier so the table loads overlap the twiddle load
+  const int e0 = g.grp_start[grp] + wave * ZLPW;
+  const int nl = min(ZLPW, g.grp_start[grp + 1] - e0);
+  const int le = lane >> 4;
+  const int sub = lane & 15;
+  int kind = 4, first = 0, lmin = 0, size = 0;
+  if (le < nl) {
+    kind = g.ent_kind[e0 + le];
+    if (kind != 4) {
+      const int rod = g.ent_rod[e0 + le];
+      first = g.rod_first[rod];
+      lmin = g.rod_lmin[rod];
+      size = g.rod_size[rod];
+    }
+  }
   const cplx<T>* twg = twtab<T>(g, 2);
   for (int i = threadIdx.x; i < np2; i += NTHR) tw[i] = twg[i];
   __syncthreads();
-  const int e0 = g.grp_start[grp] + wave * ZLPW;
-  const int nl = min(ZLPW, g.grp_start[grp + 1] - e0);
   if (nl <= 0) return;
   using PD = Pad<N1, N2>;
   const int LD = PD::row(np2);
@@ -136,32 +150,35 @@ k_zpass_bwd(GridDev g, const double* __restrict__ psi, long ld, int nst, int ban
   const double* p1 = psi + 2 * (long)n1 * ld;
   const double* p2 = psi + 2 * (long)(has2 ? n2 : n1) * ld;
 
-  // 16 lanes per line
-  const int le = lane >> 4;
-  const int sub = lane & 15;
-  if (le < nl) {
-    const int e = e0 + le;
-    const int kind = g.ent_kind[e];
-    if (kind != 4) {
-      const int rod = g.ent_rod[e];
-      const int first = g.rod_first[rod];
-      const int lmin = g.rod_lmin[rod];
-      const int size = g.rod_size[rod];
-      cplx<T>* line = buf + le * LD;
-      for (int i = sub; i < size; i += 16) {
-        const int l = lmin + i;
-        const double a = p1[2 * (first + i)];
-        const double b = p1[2 * (first + i) + 1];
-        double c = 0.0, d = 0.0;
-        if (has2) {
-          c = p2[2 * (first + i)];
-          d = p2[2 * (first + i) + 1];
+  // 16 lanes per line; KZ coefficients per lane loaded together before they are stored
+  constexpr int KZ = 4;
+  if (le < nl && kind != 4) {
+    cplx<T>* line = buf + le * LD;
+    for (int i0 = sub; i0 < size; i0 += 16 * KZ) {
+      double a[KZ], b[KZ], c[KZ], d[KZ];
+#pragma unroll
+      for (int k = 0; k < KZ; ++k) {
+        const int i = i0 + 16 * k;
+        a[k] = b[k] = c[k] = d[k] = 0.0;
+        if (i < size) {
+          a[k] = p1[2 * (first + i)];
+          b[k] = p1[2 * (first + i) + 1];
+          if (has2) {
+            c[k] = p2[2 * (first + i)];
+            d[k] = p2[2 * (first + i) + 1];
+          }
         }
+      }
+#pragma unroll
+      for (int k = 0; k < KZ; ++k) {
+        const int i = i0 + 16 * k;
+        if (i >= size) continue;
+        const int l = lmin + i;
         double pr, pi, mr, mi;  // +G and -G values
         if (MODE == SLOT_PAIR) {
-          pr = a - d; pi = b + c; mr = a + d; mi = c - b;
+          pr = a[k] - d[k]; pi = b[k] + c[k]; mr = a[k] + d[k]; mi = c[k] - b[k];
         } else {
-          pr = a; pi = b; mr = a; mi = -b;
+          pr = a[k]; pi = b[k]; mr = a[k]; mi = -b[k];
         }
         if (kind == 0) {
           if (i > 0) line[PD::idx(wrap(l, np2))] = {T(pr), T(pi)};
@@ -508,38 +525,61 @@ k_xpass2(GridDev g, cplx<T>* __restrict__ t2, int nslots, int slots_per_wg,
     }
   }
 
+  // Sphere-line element q = lane + 64k of this wavefront: row dy = q % LPW (= lane % LPW,
+  // as LPW divides 64), line lx = q / LPW.  Its LDS offset (-1 past the lines) and its T2
+  // offset within a slot do not depend on the slot: computed once here, so the slot loop
+  // issues no index-table loads (each of which would wait for all outstanding loads).
+  static_assert(64 % LPW == 0, "rows per wavefront must divide 64");
+  const bool rowok = (lane % LPW) < nyl;
+  const long sstride = (long)np2 * nlx * np1;
+  int lofs[GPL], gofs[GPL];
+#pragma unroll
+  for (int k = 0; k < GPL; ++k) {
+    const int q = lane + k * 64;
+    const int dy = q % LPW, lx = q / LPW;
+    lofs[k] = q < ng ? dy * LD + PD::idx(g.line_x[lx]) : -1;
+    gofs[k] = q < ng ? (z * nlx + lx) * np1 + y0 + dy : 0;
+  }
   cplx<T> pf[GPL];
   auto prefetch = [&](int s) {
+    const cplx<T>* src = t2 + s * sstride;
 #pragma unroll
-    for (int k = 0; k < GPL; ++k) {
-      const int q = lane + k * 64;
-      const int dy = q % LPW, lx = q / LPW;
-      pf[k] = (q < ng && dy < nyl) ? t2[(((long)s * np2 + z) * nlx + lx) * np1 + y0 + dy]
-                                   : cplx<T>{T(0), T(0)};
-    }
+    for (int k = 0; k < GPL; ++k)
+      pf[k] = (lofs[k] >= 0 && rowok) ? src[gofs[k]] : cplx<T>{T(0), T(0)};
   };
   if (QBH_X_PREFETCH && s0 < s1) prefetch(s0);
 
   // The sphere lines occupy x in [0, xpos_max] and [xneg_min, n).  Stage A input index
   // j1 in [ja, jb] is zero for every j2 and is not read; stage A' outputs there are not
-  // needed.  Only the rest of the rows outside the lines has to be cleared per slot.
+  // needed.  Only the rest of the rows outside the lines has to be cleared per slot:
+  // nz0 <= N2-1 positions below N2*ja and nz1 <= N2-1 below xneg_min, so LPW*(nz0+nz1)
+  // < LPW*2*N2 entries, ZK per lane, offsets again computed once.
   const int ja = g.xpos_max / N2 + 1;
   const int jb = g.xneg_min / N2 - 1;
   const int z0a = g.xpos_max + 1, z0b = min(N2 * ja, g.xneg_min);
   const int z1a = max(N2 * (jb + 1), z0b), z1b = g.xneg_min;
   const int nz0 = max(0, z0b - z0a), nz1 = max(0, z1b - z1a);
+  constexpr int ZK = (LPW * 2 * N2 + 63) / 64;
+  int zofs[ZK];
+#pragma unroll
+  for (int k = 0; k < ZK; ++k) {
+    const int i = lane + 64 * k;
+    const int nz = nz0 + nz1;
+    zofs[k] = -1;
+    if (i < LPW * nz) {
+      const int r = i / nz, j = i % nz;
+      zofs[k] = r * LD + PD::idx(j < nz0 ? z0a + j : z1a + j - nz0);
+    }
+  }
 
   for (int s = s0; s < s1; ++s) {
-    for (int i = lane; i < LPW * (nz0 + nz1); i += 64) {
-      const int r = i / (nz0 + nz1), k = i % (nz0 + nz1);
-      buf[r * LD + PD::idx(k < nz0 ? z0a + k : z1a + k - nz0)] = {T(0), T(0)};
-    }
+#pragma unroll
+    for (int k = 0; k < ZK; ++k)
+      if (zofs[k] >= 0) buf[zofs[k]] = {T(0), T(0)};
     if (!QBH_X_PREFETCH) prefetch(s);
 #pragma unroll
-    for (int k = 0; k < GPL; ++k) {
-      const int q = lane + k * 64;
-      if (q < ng) buf[(q % LPW) * LD + PD::idx(g.line_x[q / LPW])] = pf[k];
-    }
+    for (int k = 0; k < GPL; ++k)
+      if (lofs[k] >= 0) buf[lofs[k]] = pf[k];
     wave_sync();
     if (QBH_X_PREFETCH && s + 1 < s1) prefetch(s + 1);
 
@@ -633,10 +673,11 @@ k_xpass2(GridDev g, cplx<T>* __restrict__ t2, int nslots, int slots_per_wg,
       }
     }
     wave_sync();
-    for (int q = lane; q < ng; q += 64) {
-      const int dy = q % LPW, lx = q / LPW;
-      if (dy < nyl)
-        t2[(((long)s * np2 + z) * nlx + lx) * np1 + y0 + dy] = buf[dy * LD + PD::idx(g.line_x[lx])];
+    {
+      cplx<T>* dst = t2 + s * sstride;
+#pragma unroll
+      for (int k = 0; k < GPL; ++k)
+        if (lofs[k] >= 0 && rowok) dst[gofs[k]] = buf[lofs[k]];
     }
     wave_sync();
   }
@@ -668,11 +709,25 @@ k_zpass_fwd(GridDev g, const cplx<T>* __restrict__ t1, const double* __restrict_
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
   const int grp = blockIdx.x;
   const int slot = blockIdx.y;
+  // this wavefront's lines and the rod its 16-lane group writes out, read before the
+  // workgroup barrier so the table loads overlap the twiddle load and the transform
+  const int e0 = g.grp_start[grp] + wave * ZLPW;
+  const int nl = min(ZLPW, g.grp_start[grp + 1] - e0);
+  const int le = lane >> 4;
+  const int sub = lane & 15;
+  int kind = 4, first = 0, lmin = 0, size = 0;
+  if (le < nl) {
+    kind = g.ent_kind[e0 + le];
+    if (kind != 2 && kind != 4) {
+      const int rod = g.ent_rod[e0 + le];
+      first = g.rod_first[rod];
+      lmin = g.rod_lmin[rod];
+      size = g.rod_size[rod];
+    }
+  }
   const cplx<T>* twg = twtab<T>(g, 2);
   for (int i = threadIdx.x; i < np2; i += NTHR) tw[i] = twg[i];
   __syncthreads();
-  const int e0 = g.grp_start[grp] + wave * ZLPW;
-  const int nl = min(ZLPW, g.grp_start[grp + 1] - e0);
   if (nl <= 0) return;
   using PD = Pad<N1, N2>;
   const int LD = PD::row(np2);
@@ -696,72 +751,95 @@ k_zpass_fwd(GridDev g, const cplx<T>* __restrict__ t1, const double* __restrict_
     n2 = nst;
   }
   const bool has2 = (MODE == SLOT_PAIR) && (n2 < nst);
-  const int le = lane >> 4;
-  const int sub = lane & 15;
   if (le >= nl) return;
-  const int e = e0 + le;
-  const int kind = g.ent_kind[e];
   if (kind == 2 || kind == 4) return;  // -G image: handled with its +G partner (next line)
-  const int rod = g.ent_rod[e];
-  const int first = g.rod_first[rod];
-  const int lmin = g.rod_lmin[rod];
-  const int size = g.rod_size[rod];
   const cplx<T>* lp = buf + le * LD;
   const cplx<T>* lm = (kind == 1) ? buf + (le + 1) * LD : lp;
   double* o1 = out + 2 * (long)n1 * ldo;
   double* o2 = out + 2 * (long)(has2 ? n2 : n1) * ldo;
   const double* q1 = psi ? psi + 2 * (long)n1 * ld : nullptr;
   const double* q2 = psi ? psi + 2 * (long)(has2 ? n2 : n1) * ld : nullptr;
-  for (int i = sub; i < size; i += 16) {
-    const int l = lmin + i;
-    const int ig = first + i;
-    const cplx<T> zp = lp[PD::idx(wrap(l, np2))];
-    double r1, i1, r2 = 0.0, i2 = 0.0;
-    if (MODE == SLOT_PAIR) {
-      const cplx<T> zm = lm[PD::idx(wrap(-l, np2))];
-      const double a = double(zp.x) * fac, b = double(zp.y) * fac;
-      const double c = double(zm.x) * fac, d = double(zm.y) * fac;
-      r1 = 0.5 * (a + c);
-      i1 = 0.5 * (b - d);
-      r2 = 0.5 * (b + d);
-      i2 = 0.5 * (c - a);
-    } else {
-      r1 = double(zp.x) * fac;
-      i1 = double(zp.y) * fac;
-    }
-    if (store) {
-      o1[2 * ig] = r1;
-      o1[2 * ig + 1] = i1;
-      if (has2) {
-        o2[2 * ig] = r2;
-        o2[2 * ig + 1] = i2;
-      }
-    } else {
-      double h0 = o1[2 * ig], h1 = o1[2 * ig + 1];
+  // KZ coefficients per lane: the accumulate mode loads hpsi, psi, kin and the FP32 term
+  // of all KZ first, then combines and stores them
+  constexpr int KZ = 2;
+  for (int i0 = sub; i0 < size; i0 += 16 * KZ) {
+    double h[KZ][4], q[KZ][4], kk[KZ];
+#pragma unroll
+    for (int k = 0; k < KZ; ++k) {
+      const int i = i0 + 16 * k;
+      const int ig = first + i;
+      kk[k] = 0.0;
+#pragma unroll
+      for (int j = 0; j < 4; ++j) h[k][j] = q[k][j] = 0.0;
+      if (store || i >= size) continue;
+      h[k][0] = o1[2 * ig];
+      h[k][1] = o1[2 * ig + 1];
       if (add32) {  // FP32 projector term, in the order of a separate add before this pass
-        h0 += double(add32[2 * ((long)n1 * g.ngw + ig)]);
-        h1 += double(add32[2 * ((long)n1 * g.ngw + ig) + 1]);
+        h[k][0] += double(add32[2 * ((long)n1 * g.ngw + ig)]);
+        h[k][1] += double(add32[2 * ((long)n1 * g.ngw + ig) + 1]);
       }
       if (kin) {
-        const double k = kin[ig];
-        h0 += k * q1[2 * ig];
-        h1 += k * q1[2 * ig + 1];
+        kk[k] = kin[ig];
+        q[k][0] = q1[2 * ig];
+        q[k][1] = q1[2 * ig + 1];
       }
-      o1[2 * ig] = h0 + r1;
-      o1[2 * ig + 1] = h1 + i1;
       if (has2) {
-        double g0 = o2[2 * ig], g1 = o2[2 * ig + 1];
+        h[k][2] = o2[2 * ig];
+        h[k][3] = o2[2 * ig + 1];
         if (add32) {
-          g0 += double(add32[2 * ((long)n2 * g.ngw + ig)]);
-          g1 += double(add32[2 * ((long)n2 * g.ngw + ig) + 1]);
+          h[k][2] += double(add32[2 * ((long)n2 * g.ngw + ig)]);
+          h[k][3] += double(add32[2 * ((long)n2 * g.ngw + ig) + 1]);
         }
         if (kin) {
-          const double k = kin[ig];
-          g0 += k * q2[2 * ig];
-          g1 += k * q2[2 * ig + 1];
+          q[k][2] = q2[2 * ig];
+          q[k][3] = q2[2 * ig + 1];
         }
-        o2[2 * ig] = g0 + r2;
-        o2[2 * ig + 1] = g1 + i2;
+      }
+    }
+#pragma unroll
+    for (int k = 0; k < KZ; ++k) {
+      const int i = i0 + 16 * k;
+      if (i >= size) continue;
+      const int l = lmin + i;
+      const int ig = first + i;
+      const cplx<T> zp = lp[PD::idx(wrap(l, np2))];
+      double r1, i1, r2 = 0.0, i2 = 0.0;
+      if (MODE == SLOT_PAIR) {
+        const cplx<T> zm = lm[PD::idx(wrap(-l, np2))];
+        const double a = double(zp.x) * fac, b = double(zp.y) * fac;
+        const double c = double(zm.x) * fac, d = double(zm.y) * fac;
+        r1 = 0.5 * (a + c);
+        i1 = 0.5 * (b - d);
+        r2 = 0.5 * (b + d);
+        i2 = 0.5 * (c - a);
+      } else {
+        r1 = double(zp.x) * fac;
+        i1 = double(zp.y) * fac;
+      }
+      if (store) {
+        o1[2 * ig] = r1;
+        o1[2 * ig + 1] = i1;
+        if (has2) {
+          o2[2 * ig] = r2;
+          o2[2 * ig + 1] = i2;
+        }
+      } else {
+        double h0 = h[k][0], h1 = h[k][1];
+        if (kin) {
+          h0 += kk[k] * q[k][0];
+          h1 += kk[k] * q[k][1];
+        }
+        o1[2 * ig] = h0 + r1;
+        o1[2 * ig + 1] = h1 + i1;
+        if (has2) {
+          double g0 = h[k][2], g1 = h[k][3];
+          if (kin) {
+            g0 += kk[k] * q[k][2];
+            g1 += kk[k] * q[k][3];
+          }
+          o2[2 * ig] = g0 + r2;
+          o2[2 * ig + 1] = g1 + i2;
+        }
       }
     }
   }
