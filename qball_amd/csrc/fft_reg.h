@@ -148,8 +148,8 @@ __device__ __forceinline__ void twiddle_row(cplx<T>* a, const cplx<T>* tw, int e
 #ifndef QBH_LDS_PAD
 #define QBH_LDS_PAD 0
 #endif
-template <int N1, int N2> struct Pad {
-  static constexpr int P = (QBH_LDS_PAD && N1 > 0 && N2 % 2 == 0) ? 1 : 0;
+template <int N1, int N2, int ON = QBH_LDS_PAD> struct Pad {
+  static constexpr int P = (ON && N1 > 0 && N2 % 2 == 0) ? 1 : 0;
   static __host__ __device__ constexpr int idx(int x) { return P ? x + x / N2 : x; }
   static __host__ __device__ constexpr int row(int n) { return P ? n + n / N2 : n; }
 };

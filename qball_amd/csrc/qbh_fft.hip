@@ -42,6 +42,11 @@ template <typename T> struct Cfg;
 #ifndef QBH_XWPE
 #define QBH_XWPE 0
 #endif
+// x pass: padded LDS rows (element x at x + x/N2).  The x pass holds two workgroups per CU
+// (VGPR-bound), so the larger rows cost it no occupancy, unlike the y/z passes.
+#ifndef QBH_XLDS_PAD
+#define QBH_XLDS_PAD 1
+#endif
 // y pass: ZB z-planes x LB x-lines per workgroup, LPW lines per wavefront (ZB*LB = 4*LPW).
 // Measured on MI355X at Si512 (140^3): 16 x 1 beats 8 x 2, 4 x 4, 32 x 1 and the 8-line
 // shapes in both precisions (FP64 forward y pass 4.58 -> 4.05 ms, FP32 y passes 5.78 ->
@@ -490,27 +495,27 @@ k_xpass2(GridDev g, cplx<T>* __restrict__ t2, int nslots, int slots_per_wg,
   constexpr int n = N1 * N2;
   constexpr int TA = LPW * N2, TB = LPW * N1;
   constexpr int IA = (TA + 63) / 64, IB = (TB + 63) / 64;
-  constexpr int GPL = (LPW * n + 63) / 64;
   extern __shared__ __align__(16) unsigned char smem_raw[];
   const int np1 = g.np1, np2 = g.np2, nlx = g.nlx;
   cplx<T>* tw = reinterpret_cast<cplx<T>*>(smem_raw);
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
   const int nyb = (np1 + YB - 1) / YB;
   const int z = blockIdx.x / nyb;
-  const int y0 = (blockIdx.x % nyb) * YB + wave * LPW;
+  const int ywg = (blockIdx.x % nyb) * YB;  // the workgroup's first row
+  const int y0 = ywg + wave * LPW;
+  // rows of this wavefront; a wavefront with none still takes part in the workgroup
+  // barriers below (its transforms run on zeros and every store it would make is masked)
   const int nyl = min(LPW, np1 - y0);
   const int s0 = blockIdx.y * slots_per_wg;
   const int s1 = min(nslots, s0 + slots_per_wg);
   const cplx<T>* twg = twtab<T>(g, 0);
   for (int i = threadIdx.x; i < n; i += NTHR) tw[i] = twg[i];
   __syncthreads();
-  if (nyl <= 0) return;
-  using PD = Pad<N1, N2>;
+  using PD = Pad<N1, N2, QBH_XLDS_PAD>;
   constexpr int LD = PD::row(n);
   constexpr int S = N2 + PD::P;
   cplx<T>* buf = tw + n + wave * LPW * LD;
   const long grow0 = ((long)z * np1 + y0) * n;
-  const int ng = nlx * LPW;
 
   T vr[IB][N2];
   double acc[IB][N2];
@@ -525,26 +530,30 @@ k_xpass2(GridDev g, cplx<T>* __restrict__ t2, int nslots, int slots_per_wg,
     }
   }
 
-  // Sphere-line element q = lane + 64k of this wavefront: row dy = q % LPW (= lane % LPW,
-  // as LPW divides 64), line lx = q / LPW.  Its LDS offset (-1 past the lines) and its T2
-  // offset within a slot do not depend on the slot: computed once here, so the slot loop
-  // issues no index-table loads (each of which would wait for all outstanding loads).
-  static_assert(64 % LPW == 0, "rows per wavefront must divide 64");
-  const bool rowok = (lane % LPW) < nyl;
+  // The workgroup moves the sphere lines of its YB rows together: element q = tid + NTHR*k
+  // is row dy = q % YB (= tid % YB, as YB divides NTHR) of line lx = q / YB, so one load
+  // instruction covers YB consecutive rows of NTHR/YB lines (YB*sizeof(cplx) contiguous
+  // bytes per line) instead of each wavefront's LPW rows.  LDS offset (relative to the
+  // first row buffer; -1 past the lines) and T2 offset within a slot are computed once.
+  static_assert(NTHR % YB == 0, "rows per workgroup must divide the workgroup size");
+  constexpr int GPW = (YB * n + NTHR - 1) / NTHR;
+  const bool rowok = ywg + (int)(threadIdx.x % YB) < np1;
   const long sstride = (long)np2 * nlx * np1;
-  int lofs[GPL], gofs[GPL];
+  const int ngw = nlx * YB;
+  cplx<T>* rows = tw + n;  // row buffers of all wavefronts, row dy at dy * LD
+  int lofs[GPW], gofs[GPW];
 #pragma unroll
-  for (int k = 0; k < GPL; ++k) {
-    const int q = lane + k * 64;
-    const int dy = q % LPW, lx = q / LPW;
-    lofs[k] = q < ng ? dy * LD + PD::idx(g.line_x[lx]) : -1;
-    gofs[k] = q < ng ? (z * nlx + lx) * np1 + y0 + dy : 0;
+  for (int k = 0; k < GPW; ++k) {
+    const int q = threadIdx.x + k * NTHR;
+    const int dy = q % YB, lx = q / YB;
+    lofs[k] = q < ngw ? dy * LD + PD::idx(g.line_x[lx]) : -1;
+    gofs[k] = q < ngw ? (z * nlx + lx) * np1 + ywg + dy : 0;
   }
-  cplx<T> pf[GPL];
+  cplx<T> pf[GPW];
   auto prefetch = [&](int s) {
     const cplx<T>* src = t2 + s * sstride;
 #pragma unroll
-    for (int k = 0; k < GPL; ++k)
+    for (int k = 0; k < GPW; ++k)
       pf[k] = (lofs[k] >= 0 && rowok) ? src[gofs[k]] : cplx<T>{T(0), T(0)};
   };
   if (QBH_X_PREFETCH && s0 < s1) prefetch(s0);
@@ -572,15 +581,18 @@ k_xpass2(GridDev g, cplx<T>* __restrict__ t2, int nslots, int slots_per_wg,
     }
   }
 
+  // Per slot: clear (own rows) and scatter (all rows), barrier, transforms on own rows,
+  // barrier, gather (all rows), barrier.  Cleared positions are never line positions, so
+  // the clear needs no barrier against the scatter or the previous gather.
   for (int s = s0; s < s1; ++s) {
 #pragma unroll
     for (int k = 0; k < ZK; ++k)
       if (zofs[k] >= 0) buf[zofs[k]] = {T(0), T(0)};
     if (!QBH_X_PREFETCH) prefetch(s);
 #pragma unroll
-    for (int k = 0; k < GPL; ++k)
-      if (lofs[k] >= 0) buf[lofs[k]] = pf[k];
-    wave_sync();
+    for (int k = 0; k < GPW; ++k)
+      if (lofs[k] >= 0) rows[lofs[k]] = pf[k];
+    __syncthreads();
     if (QBH_X_PREFETCH && s + 1 < s1) prefetch(s + 1);
 
     // stage A, backward
@@ -652,7 +664,7 @@ k_xpass2(GridDev g, cplx<T>* __restrict__ t2, int nslots, int slots_per_wg,
       }
     }
     if (OP == X_DENSITY) {
-      wave_sync();
+      __syncthreads();  // the next scatter overwrites line positions read above
       continue;
     }
     wave_sync();
@@ -672,14 +684,14 @@ k_xpass2(GridDev g, cplx<T>* __restrict__ t2, int nslots, int slots_per_wg,
           if (m1 < ja || m1 > jb) row[S * m1] = a[m1];
       }
     }
-    wave_sync();
+    __syncthreads();
     {
       cplx<T>* dst = t2 + s * sstride;
 #pragma unroll
-      for (int k = 0; k < GPL; ++k)
-        if (lofs[k] >= 0 && rowok) dst[gofs[k]] = buf[lofs[k]];
+      for (int k = 0; k < GPW; ++k)
+        if (lofs[k] >= 0 && rowok) dst[gofs[k]] = rows[lofs[k]];
     }
-    wave_sync();
+    __syncthreads();
   }
   if (OP == X_DENSITY) {
 #pragma unroll
@@ -865,11 +877,11 @@ static void reg_factors(int n, int& f1, int& f2) {
     default: break;
   }
 }
-// LDS row length (Pad<N1,N2>::row) of an edge of length n
-static int lds_row(int n) {
+// LDS row length (Pad<N1,N2,pad>::row) of an edge of length n
+static int lds_row(int n, int pad = QBH_LDS_PAD) {
   int f1, f2;
   reg_factors(n, f1, f2);
-  return (QBH_LDS_PAD && f1 > 0 && f2 % 2 == 0) ? n + n / f2 : n;
+  return (pad && f1 > 0 && f2 % 2 == 0) ? n + n / f2 : n;
 }
 template <typename T>
 static size_t lds_z(const GridDev& g) {
@@ -1065,7 +1077,8 @@ static int xpass_t(Ctx* ctx, Basis* b, const PipelineArgs& a, const double* in, 
   const int yb = rows * NWAVE;
   const int nyb = (g.np1 + yb - 1) / yb;
   const size_t lds =
-      sizeof(cplx<T>) * ((size_t)g.np0 + (size_t)NWAVE * rows * (reg ? lds_row(g.np0) : g.np0));
+      sizeof(cplx<T>) *
+      ((size_t)g.np0 + (size_t)NWAVE * rows * (reg ? lds_row(g.np0, QBH_XLDS_PAD) : g.np0));
   int spw = (OP == X_DENSITY) ? a.nslots
             : b->tune.x_spw > 0 ? std::min(b->tune.x_spw, a.nslots)
                                 : x_slots_per_wg(g, a.nslots, yb);
