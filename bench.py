@@ -235,6 +235,20 @@ def main():
         e32 = band_energy(h32)
         rel = float(np.linalg.norm(h32 - h64) / np.linalg.norm(h64))
         del h32
+        # density L1 error of the FP32-bulk transforms (FP64 band sum) against FP64:
+        # (Omega/N) sum |rho32 - rho64| in electrons, and relative to N_e
+        N = int(np.prod(sysm.np))
+        rho = DeviceArray(8 * N)
+        dens = {}
+        for name, fl in (("fp64", 0), ("fp32", QBH_FP32)):
+            rho.zero()
+            basis.density(psi.ptr, ngw, nst, sysm.occ, 1.0, sysm.omega, rho.ptr, flags=fl)
+            ctx.sync()
+            dens[name] = rho.to_numpy()
+        rho.close()
+        dl1 = float(np.abs(dens["fp32"] - dens["fp64"]).sum() * sysm.omega / N)
+        ne = float(dens["fp64"].sum() * sysm.omega / N)
+        del dens
         apply(builtin)
         _, k32c = timed(QBH_ALL | QBH_FP32, char_steps)
         k32 = {k: {"avg_ms": ms / cnt, "launches_per_step": cnt / char_steps}
@@ -242,6 +256,8 @@ def main():
         fp32 = {"value": nst / dt32, "ms_per_step": dt32 * 1e3, "speedup_vs_fp64": dt / dt32,
                 "hpsi_rel_err_vs_fp64": rel,
                 "band_energy_diff_per_atom_Ha": abs(e32 - e64) / sysm.na,
+                "density_L1_vs_fp64_electrons": dl1, "density_L1_rel": dl1 / ne,
+                "nelectrons": ne,
                 "tuning": sched32, "kernels_sequential_builtin": k32}
     del h64
 

@@ -316,12 +316,19 @@ int qbh::hpsi_impl(Ham* h, const double* psi, long ld, int nst, const double* oc
   if (psi == hpsi) return fail(c, QBH_ERR_ARG, "psi and hpsi must not alias");
   const bool fp32 = (flags & QBH_FP32) != 0;
   const int prec = fp32 ? PREC_FP32 : PREC_FP64;
-  if (!(flags & QBH_ACCUMULATE)) fill0(c, hpsi, 2 * ldh * (long)nst);  // dwf.c().clear()
   if (enl) *enl = 0.0;
   const bool do_nl = (flags & QBH_NONLOCAL) && h->P > 0;
   const bool do_loc = (flags & QBH_LOCAL) != 0;
   const bool kin = (flags & QBH_KINETIC) != 0;
   if (do_loc && !h->v) return fail(c, QBH_ERR_ARG, "local potential not set");
+  // dwf.c().clear(): with no padding rows (ldh == ngw) the first term written overwrites
+  // instead -- FP64: the projector product (beta 0); FP32 with the local part: the forward
+  // z pass (hpsi = projector + kinetic + local terms, with the projector term deferred)
+  const bool fresh = !(flags & QBH_ACCUMULATE) && ldh == ngw;
+  const bool nl_overwrites = fresh && !fp32 && do_nl;
+  const bool z_overwrites = fresh && fp32 && do_loc;
+  if (!(flags & QBH_ACCUMULATE) && !nl_overwrites && !z_overwrites)
+    fill0(c, hpsi, 2 * ldh * (long)nst);
   hipStream_t main = c->stream;
   // Order of accumulation into hpsi: non-local (EnergyFunctional.cc:1500), then kinetic,
   // then local (:1654-1695).  The projector GEMMs run on the side stream, concurrently
@@ -340,7 +347,7 @@ int qbh::hpsi_impl(Ham* h, const double* psi, long ld, int nst, const double* oc
     double dummy = 0.0;
     // FP32: the forward z pass adds the projector term while it accumulates into hpsi
     const int r = nl_apply(h, psi, ld, nst, occ, hpsi, ldh, fp32 ? 1 : 0, enl ? &dummy : nullptr,
-                           (fp32 && do_loc) ? &add32 : nullptr);
+                           (fp32 && do_loc) ? &add32 : nullptr, nl_overwrites);
     if (overlap) {
       hipEventRecord(c->ev_join, c->side);
       c->stream = main;
@@ -364,7 +371,8 @@ int qbh::hpsi_impl(Ham* h, const double* psi, long ld, int nst, const double* oc
       a.add32 = add32;
       if (int r = pipeline_bwd_yz(c, b, a)) return r;
       if (int r = pipeline_x_vmul(c, b, a, h->v)) return r;
-      return pipeline_fwd_zy_accumulate(c, b, a, hpsi, ldh, kin ? h->kin : nullptr, 0);
+      return pipeline_fwd_zy_accumulate(c, b, a, hpsi, ldh, kin ? h->kin : nullptr,
+                                        z_overwrites ? 2 : 0);
     });
     if (join) hipStreamWaitEvent(main, join, 0);  // also on early exit
     if (r) return r;

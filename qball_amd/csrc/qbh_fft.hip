@@ -708,7 +708,10 @@ k_xpass2(GridDev g, cplx<T>* __restrict__ t2, int nslots, int slots_per_wg,
 
 // ------------------------------------------------------------------------------------
 // z pass, forward: T1 columns -> z transform (exp(-i)) -> x 1/N -> unpack -> output.
-// out[n][G] = value (store) or out[n][G] = (out + kin[G]*psi[n][G]) + value.
+// store 0: out[n][G] = ((out + add32) + kin[G]*psi[n][G]) + value;
+// store 1: out[n][G] = value;
+// store 2: out[n][G] = (add32 + kin[G]*psi[n][G]) + value, out not read (a cleared output:
+//          the same sums as store 0 on zeros).
 template <typename T, int MODE, int NMAX, int N1, int N2>
 __global__ void __launch_bounds__(NTHR)
 k_zpass_fwd(GridDev g, const cplx<T>* __restrict__ t1, const double* __restrict__ psi, long ld,
@@ -783,9 +786,11 @@ k_zpass_fwd(GridDev g, const cplx<T>* __restrict__ t1, const double* __restrict_
       kk[k] = 0.0;
 #pragma unroll
       for (int j = 0; j < 4; ++j) h[k][j] = q[k][j] = 0.0;
-      if (store || i >= size) continue;
-      h[k][0] = o1[2 * ig];
-      h[k][1] = o1[2 * ig + 1];
+      if (store == 1 || i >= size) continue;
+      if (store == 0) {
+        h[k][0] = o1[2 * ig];
+        h[k][1] = o1[2 * ig + 1];
+      }
       if (add32) {  // FP32 projector term, in the order of a separate add before this pass
         h[k][0] += double(add32[2 * ((long)n1 * g.ngw + ig)]);
         h[k][1] += double(add32[2 * ((long)n1 * g.ngw + ig) + 1]);
@@ -796,8 +801,10 @@ k_zpass_fwd(GridDev g, const cplx<T>* __restrict__ t1, const double* __restrict_
         q[k][1] = q1[2 * ig + 1];
       }
       if (has2) {
-        h[k][2] = o2[2 * ig];
-        h[k][3] = o2[2 * ig + 1];
+        if (store == 0) {
+          h[k][2] = o2[2 * ig];
+          h[k][3] = o2[2 * ig + 1];
+        }
         if (add32) {
           h[k][2] += double(add32[2 * ((long)n2 * g.ngw + ig)]);
           h[k][3] += double(add32[2 * ((long)n2 * g.ngw + ig) + 1]);
@@ -828,7 +835,7 @@ k_zpass_fwd(GridDev g, const cplx<T>* __restrict__ t1, const double* __restrict_
         r1 = double(zp.x) * fac;
         i1 = double(zp.y) * fac;
       }
-      if (store) {
+      if (store == 1) {
         o1[2 * ig] = r1;
         o1[2 * ig + 1] = i1;
         if (has2) {

@@ -60,9 +60,11 @@ int hpsi_impl(Ham* h, const double* psi, long ld, int nst, const double* occ, do
 int expo_impl(Ham* h, const double* in, long ldin, double* outp, long ldout, int nst, double dt,
               int order, int flags, double* krylov);
 // defer_add32 (nullable, FP32 only): instead of adding the FP32 product to hpsi, return
-// it (compact, 2*ngw floats per band) for the forward z pass to add
+// it (compact, 2*ngw floats per band) for the forward z pass to add.  overwrite (FP64):
+// hpsi = beta F instead of hpsi += beta F (rows 0..2*ngw-1 of every band)
 int nl_apply(Ham* h, const double* psi, long ld, int nst, const double* occ, double* hpsi,
-             long ldh, int fp32, double* enl, const float** defer_add32 = nullptr);
+             long ldh, int fp32, double* enl, const float** defer_add32 = nullptr,
+             bool overwrite = false);
 double nl_read_enl(Ham* h, hipStream_t s);
 // number of hipBLASLt algorithms listed for the fnl (which 0) or enl (which 1) GEMM of an
 // Hψ call with these arguments (0: hipBLASLt does not cover it)

@@ -142,7 +142,8 @@ struct PipelineArgs {
 int pipeline_bwd_yz(Ctx* ctx, Basis* b, const PipelineArgs& a);
 int pipeline_fwd_zy_accumulate(Ctx* ctx, Basis* b, const PipelineArgs& a, double* hpsi,
                                long ldh, const double* kin /*0.5|k+G|^2 or null*/,
-                               int store /*1: hpsi = value instead of +=*/);
+                               int store /*0: hpsi += kinetic, FP32 term, value; 1: hpsi =
+                                           value; 2: hpsi = kinetic + FP32 term + value*/);
 int pipeline_x_vmul(Ctx* ctx, Basis* b, const PipelineArgs& a, const double* v);
 int pipeline_x_store(Ctx* ctx, Basis* b, const PipelineArgs& a, double* f /*complex grid*/);
 int pipeline_x_load(Ctx* ctx, Basis* b, const PipelineArgs& a, const double* f, int real_input,

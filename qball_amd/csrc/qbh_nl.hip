@@ -461,7 +461,7 @@ int nl_lt_candidates(Ham* h, int which, int nst, long ld, long ldh, int fp32) {
 }
 
 int nl_apply(Ham* h, const double* psi, long ld, int nst, const double* occ_host, double* hpsi,
-             long ldh, int fp32, double* enl, const float** defer_add32) {
+             long ldh, int fp32, double* enl, const float** defer_add32, bool overwrite) {
   Ctx* c = h->ctx;
   if (h->P == 0) {
     if (enl) *enl = 0.0;
@@ -542,12 +542,13 @@ int nl_apply(Ham* h, const double* psi, long ld, int nst, const double* occ_host
                                                                h->col_fac, occ, h->red);
     }
     TimedLaunch tl(c, "enl_gemm");
+    const double beta = overwrite ? 0.0 : 1.0;
     if (eplan) {
-      if (int r = lt_gemm<double>(c, eplan, ealgo, anl, F, hpsi, 1.0)) return r;
+      if (int r = lt_gemm<double>(c, eplan, ealgo, anl, F, hpsi, beta)) return r;
     } else if (int r = blas_ok(c,
                                gemm_sb(bh, rocblas_operation_none, rocblas_operation_none,
                                        (int)Kelem, nst, P, anl, Kelem, 0, F, P, 0, hpsi,
-                                       real ? 2 * ldh : ldh, 0, 1, cplx, 1.0),
+                                       real ? 2 * ldh : ldh, 0, 1, cplx, beta),
                                "enl gemm")) {
       return r;
     }

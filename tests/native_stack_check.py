@@ -4,7 +4,8 @@ A process that never imports torch, so libqbh.so runs on the HIP runtime, rocBLA
 hipBLASLt it was linked against (as in a C++ host), not on the copies bundled with torch.
 Checks Hψ of an 8-atom Si cell (Γ, and a complex basis) against the oracle for the built-in
 projector GEMM choice and for every algorithm hipBLASLt lists, in FP64 and in the FP32
-form.  Prints one JSON line with the errors.
+form, into an output buffer full of NaN (which Hψ without accumulation must not read).
+Prints one JSON line with the errors.
 """
 import json
 import os
@@ -45,7 +46,10 @@ def main():
         psi = DeviceArray.from_numpy(np.ascontiguousarray(s.psi.T).view(np.float64).ravel())
         hp = DeviceArray(psi.nbytes)
 
+        nan = np.full(psi.nbytes // 8, np.nan)
+
         def run(flags):
+            hp.upload(nan)  # a cleared output is never read: NaN must not reach the result
             ham.hpsi(psi.ptr, ngw, s.nst, s.occ, hp.ptr, ngw, flags=flags)
             ctx.sync()
             got = hp.to_numpy().view(np.complex128).reshape(s.nst, ngw).T

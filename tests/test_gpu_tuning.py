@@ -171,3 +171,32 @@ def test_batch_create_reports_disabled(gpu_ctx):
     from qball_amd import Batch, QbhError
     with pytest.raises(QbhError, match="disabled"):
         Batch(gpu_ctx, [(SimpleNamespace(h=ctypes.c_void_p()), 16, 1, 1, 32, 1)])
+
+
+@pytest.mark.parametrize("flags,tol", [(0x7, TOL), (0x17, 2e-6), (0x5, TOL), (0x3, TOL),
+                                       (0x13, 2e-6)])
+def test_cleared_output_is_never_read(gpu_ctx, flags, tol):
+    """Without QBH_ACCUMULATE the output starts cleared: an hpsi full of NaN gives the same
+    Hψ as a zeroed one, whichever term writes it first (projector product, forward z pass or
+    the clear itself), for both projector GEMM libraries."""
+    import torch
+    c = _cell(gpu_ctx, "si8", None, seed=6)
+    s = c["sysm"]
+    ham = c["ham"]
+    cd = to_dev(s.psi)
+    ngw = c["basis"].ngw
+    for algo in (0, 2):
+        ham.set_tuning("fnl_algo", algo)
+        ham.set_tuning("enl_algo", algo)
+        ref = torch.zeros_like(cd)
+        ham.hpsi(cd.data_ptr(), ngw, s.nst, s.occ, ref.data_ptr(), ngw, flags=flags)
+        got = torch.full_like(cd, float("nan"))
+        ham.hpsi(cd.data_ptr(), ngw, s.nst, s.occ, got.data_ptr(), ngw, flags=flags)
+        gpu_ctx.sync()
+        assert bool(torch.isfinite(got).all()), (flags, algo)
+        a, b = from_dev(got, ngw, s.nst), from_dev(ref, ngw, s.nst)
+        assert rel_err(a, b) <= 1e-15, (flags, algo)
+        if flags & 0x4 and flags & 0x2 and flags & 0x1:
+            assert rel_err(a, _ref(c)) <= tol, (flags, algo)
+    ham.close()
+    c["basis"].close()
