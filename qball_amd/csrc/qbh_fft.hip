@@ -36,11 +36,23 @@ template <typename T> struct Cfg;
 #ifndef QBH_XR32
 #define QBH_XR32 4
 #endif
-#ifndef QBH_X_PREFETCH
-#define QBH_X_PREFETCH 1
+// x pass: slots whose sphere lines are loaded ahead (register buffers of GPW elements
+// each).  Measured on MI355X at Si512: two slots ahead slowed the FP32 x pass from 4.26
+// to 4.60 ms per Hψ at three waves per SIMD (register spills) and to 5.57 ms at two; FP64
+// has no registers to spare.
+#ifndef QBH_XPF64
+#define QBH_XPF64 1
 #endif
-#ifndef QBH_XWPE
-#define QBH_XWPE 0
+#ifndef QBH_XPF32
+#define QBH_XPF32 1
+#endif
+// x pass: minimum waves per SIMD the register allocation must allow (the counts the
+// kernels reach without spilling: FP32 three, FP64 two)
+#ifndef QBH_XWPE64
+#define QBH_XWPE64 2
+#endif
+#ifndef QBH_XWPE32
+#define QBH_XWPE32 3
 #endif
 // x pass: padded LDS rows (element x at x + x/N2).  The x pass holds two workgroups per CU
 // (VGPR-bound), so the larger rows cost it no occupancy, unlike the y/z passes.
@@ -65,12 +77,16 @@ template <typename T> struct Cfg;
 #endif
 template <> struct Cfg<double> {
   static constexpr int XR = QBH_XR64;  // x pass: rows per wavefront (64 B row segments)
+  static constexpr int XPF = QBH_XPF64;  // x pass: slots of sphere lines in flight
+  static constexpr int XWPE = QBH_XWPE64;  // x pass: minimum waves per SIMD asked of the compiler
   static constexpr int ZB = QBH_YZB64;  // y pass: z-planes per workgroup (column segments)
   static constexpr int LB = QBH_YLB64;  // y pass: x-lines per workgroup
   static constexpr int LPW = ZB * LB / 4;
 };
 template <> struct Cfg<float> {
   static constexpr int XR = QBH_XR32;
+  static constexpr int XPF = QBH_XPF32;
+  static constexpr int XWPE = QBH_XWPE32;
   static constexpr int ZB = QBH_YZB32;
   static constexpr int LB = QBH_YLB32;
   static constexpr int LPW = ZB * LB / 4;
@@ -484,9 +500,7 @@ k_xpass(GridDev g, cplx<T>* __restrict__ t2, int nslots, int slots_per_wg,
 // multiply touches each element once, in registers.
 template <typename T, int OP, int N1, int N2>
 __global__ void __launch_bounds__(NTHR)
-#if QBH_XWPE > 0
-__attribute__((amdgpu_waves_per_eu(QBH_XWPE)))
-#endif
+__attribute__((amdgpu_waves_per_eu(Cfg<T>::XWPE)))
 k_xpass2(GridDev g, cplx<T>* __restrict__ t2, int nslots, int slots_per_wg,
          const double* __restrict__ v, double* __restrict__ out, const double* __restrict__ dens,
          int cplx_density) {
@@ -549,14 +563,19 @@ k_xpass2(GridDev g, cplx<T>* __restrict__ t2, int nslots, int slots_per_wg,
     lofs[k] = q < ngw ? dy * LD + PD::idx(g.line_x[lx]) : -1;
     gofs[k] = q < ngw ? (z * nlx + lx) * np1 + ywg + dy : 0;
   }
-  cplx<T> pf[GPW];
-  auto prefetch = [&](int s) {
+  // the sphere lines of the next XPF slots are in flight in registers while a slot
+  // transforms (buffer p of slot s is refilled with slot s + XPF)
+  constexpr int XPF = Cfg<T>::XPF;
+  cplx<T> pf[XPF][GPW];
+  auto prefetch = [&](int s, cplx<T>(&p)[GPW]) {
     const cplx<T>* src = t2 + s * sstride;
 #pragma unroll
     for (int k = 0; k < GPW; ++k)
-      pf[k] = (lofs[k] >= 0 && rowok) ? src[gofs[k]] : cplx<T>{T(0), T(0)};
+      p[k] = (lofs[k] >= 0 && rowok) ? src[gofs[k]] : cplx<T>{T(0), T(0)};
   };
-  if (QBH_X_PREFETCH && s0 < s1) prefetch(s0);
+#pragma unroll
+  for (int d = 0; d < XPF; ++d)
+    if (s0 + d < s1) prefetch(s0 + d, pf[d]);
 
   // The sphere lines occupy x in [0, xpos_max] and [xneg_min, n).  Stage A input index
   // j1 in [ja, jb] is zero for every j2 and is not read; stage A' outputs there are not
@@ -584,16 +603,15 @@ k_xpass2(GridDev g, cplx<T>* __restrict__ t2, int nslots, int slots_per_wg,
   // Per slot: clear (own rows) and scatter (all rows), barrier, transforms on own rows,
   // barrier, gather (all rows), barrier.  Cleared positions are never line positions, so
   // the clear needs no barrier against the scatter or the previous gather.
-  for (int s = s0; s < s1; ++s) {
+  auto run_slot = [&](int s, cplx<T>(&p)[GPW]) {
 #pragma unroll
     for (int k = 0; k < ZK; ++k)
       if (zofs[k] >= 0) buf[zofs[k]] = {T(0), T(0)};
-    if (!QBH_X_PREFETCH) prefetch(s);
 #pragma unroll
     for (int k = 0; k < GPW; ++k)
-      if (lofs[k] >= 0) rows[lofs[k]] = pf[k];
+      if (lofs[k] >= 0) rows[lofs[k]] = p[k];
     __syncthreads();
-    if (QBH_X_PREFETCH && s + 1 < s1) prefetch(s + 1);
+    if (s + XPF < s1) prefetch(s + XPF, p);
 
     // stage A, backward
 #pragma unroll
@@ -665,7 +683,7 @@ k_xpass2(GridDev g, cplx<T>* __restrict__ t2, int nslots, int slots_per_wg,
     }
     if (OP == X_DENSITY) {
       __syncthreads();  // the next scatter overwrites line positions read above
-      continue;
+      return;
     }
     wave_sync();
     // stage A', forward: natural order
@@ -692,6 +710,11 @@ k_xpass2(GridDev g, cplx<T>* __restrict__ t2, int nslots, int slots_per_wg,
         if (lofs[k] >= 0 && rowok) dst[gofs[k]] = rows[lofs[k]];
     }
     __syncthreads();
+  };
+  for (int s = s0; s < s1; s += XPF) {
+#pragma unroll
+    for (int d = 0; d < XPF; ++d)
+      if (s + d < s1) run_slot(s + d, pf[d]);
   }
   if (OP == X_DENSITY) {
 #pragma unroll
