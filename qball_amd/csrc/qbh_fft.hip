@@ -498,7 +498,7 @@ k_xpass(GridDev g, cplx<T>* __restrict__ t2, int nslots, int slots_per_wg,
 //   stage A' (fwd): DFT_N1 over stride-N2 elements -> natural order
 // Three LDS round trips per slot instead of one per radix pass, and the x-dependent
 // multiply touches each element once, in registers.
-template <typename T, int OP, int N1, int N2>
+template <typename T, int OP, int N1, int N2, int GPW>
 __global__ void __launch_bounds__(NTHR)
 __attribute__((amdgpu_waves_per_eu(Cfg<T>::XWPE)))
 k_xpass2(GridDev g, cplx<T>* __restrict__ t2, int nslots, int slots_per_wg,
@@ -547,21 +547,27 @@ k_xpass2(GridDev g, cplx<T>* __restrict__ t2, int nslots, int slots_per_wg,
   // The workgroup moves the sphere lines of its YB rows together: element q = tid + NTHR*k
   // is row dy = q % YB (= tid % YB, as YB divides NTHR) of line lx = q / YB, so one load
   // instruction covers YB consecutive rows of NTHR/YB lines (YB*sizeof(cplx) contiguous
-  // bytes per line) instead of each wavefront's LPW rows.  LDS offset (relative to the
-  // first row buffer; -1 past the lines) and T2 offset within a slot are computed once.
+  // bytes per line) instead of each wavefront's LPW rows.  GPW (a launch parameter) covers
+  // nlx*YB elements.  LDS offset (relative to the first row buffer) and T2 offset within a
+  // slot are computed once, and the loads and LDS stores are unconditional: an element past
+  // the lines loads element 0 of the slot and stores it to this thread's dump entry after
+  // the row buffers; a row past np1 loads element 0 into its own (never stored) row.
   static_assert(NTHR % YB == 0, "rows per workgroup must divide the workgroup size");
-  constexpr int GPW = (YB * n + NTHR - 1) / NTHR;
   const bool rowok = ywg + (int)(threadIdx.x % YB) < np1;
   const long sstride = (long)np2 * nlx * np1;
   const int ngw = nlx * YB;
   cplx<T>* rows = tw + n;  // row buffers of all wavefronts, row dy at dy * LD
+  constexpr int DUMP = NWAVE * LPW * LD;  // + threadIdx.x: one LDS entry per thread
   int lofs[GPW], gofs[GPW];
+  unsigned gok = 0;  // bit k: element k is a line entry of a row below np1
 #pragma unroll
   for (int k = 0; k < GPW; ++k) {
     const int q = threadIdx.x + k * NTHR;
     const int dy = q % YB, lx = q / YB;
-    lofs[k] = q < ngw ? dy * LD + PD::idx(g.line_x[lx]) : -1;
-    gofs[k] = q < ngw ? (z * nlx + lx) * np1 + ywg + dy : 0;
+    const bool in = q < ngw;
+    lofs[k] = in ? dy * LD + PD::idx(g.line_x[lx]) : DUMP + (int)threadIdx.x;
+    gofs[k] = (in && rowok) ? (z * nlx + lx) * np1 + ywg + dy : 0;
+    if (in && rowok) gok |= 1u << k;
   }
   // the sphere lines of the next XPF slots are in flight in registers while a slot
   // transforms (buffer p of slot s is refilled with slot s + XPF)
@@ -570,8 +576,11 @@ k_xpass2(GridDev g, cplx<T>* __restrict__ t2, int nslots, int slots_per_wg,
   auto prefetch = [&](int s, cplx<T>(&p)[GPW]) {
     const cplx<T>* src = t2 + s * sstride;
 #pragma unroll
-    for (int k = 0; k < GPW; ++k)
-      p[k] = (lofs[k] >= 0 && rowok) ? src[gofs[k]] : cplx<T>{T(0), T(0)};
+    for (int k = 0; k < GPW; ++k) {
+      const cplx<T> e = src[gofs[k]];
+      p[k].x = e.x;
+      p[k].y = e.y;
+    }
   };
 #pragma unroll
   for (int d = 0; d < XPF; ++d)
@@ -608,8 +617,7 @@ k_xpass2(GridDev g, cplx<T>* __restrict__ t2, int nslots, int slots_per_wg,
     for (int k = 0; k < ZK; ++k)
       if (zofs[k] >= 0) buf[zofs[k]] = {T(0), T(0)};
 #pragma unroll
-    for (int k = 0; k < GPW; ++k)
-      if (lofs[k] >= 0) rows[lofs[k]] = p[k];
+    for (int k = 0; k < GPW; ++k) rows[lofs[k]] = cplx<T>{p[k].x, p[k].y};
     __syncthreads();
     if (s + XPF < s1) prefetch(s + XPF, p);
 
@@ -707,7 +715,7 @@ k_xpass2(GridDev g, cplx<T>* __restrict__ t2, int nslots, int slots_per_wg,
       cplx<T>* dst = t2 + s * sstride;
 #pragma unroll
       for (int k = 0; k < GPW; ++k)
-        if (lofs[k] >= 0 && rowok) dst[gofs[k]] = rows[lofs[k]];
+        if ((gok >> k) & 1u) dst[gofs[k]] = rows[lofs[k]];
     }
     __syncthreads();
   };
@@ -1079,11 +1087,21 @@ static bool launch_x2(Ctx* ctx, const GridDev& g, dim3 grid, size_t lds, cplx<T>
   if constexpr (OP != X_VMUL && OP != X_DENSITY) {
     return false;
   } else {
-#define QBH_X2(NN, A, B)                                                                  \
-  case NN:                                                                                  \
-    k_xpass2<T, OP, A, B><<<grid, NTHR, lds, ctx->stream>>>(g, t2, nslots, spw, in, out, dens, \
-                                                            cd);                            \
-    return true;
+  // line elements per thread: a short count for spheres up to about half the edge (the
+  // usual Hψ grid), the full edge otherwise
+  constexpr int YB = Cfg<T>::XR * NWAVE;
+  const int need = (g.nlx * YB + NTHR - 1) / NTHR;
+#define QBH_X2(NN, A, B)                                                                      \
+  case NN: {                                                                                    \
+    constexpr int GS = (YB * (NN / 2 + 8) + NTHR - 1) / NTHR, GF = (YB * NN + NTHR - 1) / NTHR; \
+    if (need <= GS)                                                                             \
+      k_xpass2<T, OP, A, B, GS><<<grid, NTHR, lds, ctx->stream>>>(g, t2, nslots, spw, in, out, \
+                                                                  dens, cd);                    \
+    else                                                                                        \
+      k_xpass2<T, OP, A, B, GF><<<grid, NTHR, lds, ctx->stream>>>(g, t2, nslots, spw, in, out, \
+                                                                  dens, cd);                    \
+    return true;                                                                                \
+  }
   switch (g.np0) {
     QBH_X2(28, 7, 4)
     QBH_X2(36, 6, 6)
@@ -1106,9 +1124,11 @@ static int xpass_t(Ctx* ctx, Basis* b, const PipelineArgs& a, const double* in, 
   const int rows = reg ? Cfg<T>::XR : Cfg<T>::LPW;  // rows per wavefront
   const int yb = rows * NWAVE;
   const int nyb = (g.np1 + yb - 1) / yb;
+  // twiddles, row buffers and (register x pass) one dump entry per thread
   const size_t lds =
-      sizeof(cplx<T>) *
-      ((size_t)g.np0 + (size_t)NWAVE * rows * (reg ? lds_row(g.np0, QBH_XLDS_PAD) : g.np0));
+      sizeof(cplx<T>) * ((size_t)g.np0 + (size_t)NWAVE * rows *
+                                             (reg ? lds_row(g.np0, QBH_XLDS_PAD) : g.np0) +
+                         (reg ? NTHR : 0));
   int spw = (OP == X_DENSITY) ? a.nslots
             : b->tune.x_spw > 0 ? std::min(b->tune.x_spw, a.nslots)
                                 : x_slots_per_wg(g, a.nslots, yb);
