@@ -295,6 +295,16 @@ k_ypass_fwd(GridDev g, const cplx<T>* __restrict__ t2, cplx<T>* __restrict__ t1)
   const int lx1 = min(lx0 + LB, nlx);
   const cplx<T>* twg = twtab<T>(g, 1);
   for (int i = threadIdx.x; i < np1; i += NTHR) tw[i] = twg[i];
+  // LDS offset (without the z-plane term) of each column of this workgroup's lines, staged
+  // once so the column scatter below waits on no index-table load
+  int* cof = reinterpret_cast<int*>(buf + ZB * LB * LD);
+  const int c0 = g.line_start[lx0], c1 = g.line_start[lx1];
+  for (int i = threadIdx.x; i < c1 - c0; i += NTHR) {
+    const int c = c0 + i;
+    int dl = 0;
+    while (lx0 + dl + 1 < lx1 && c >= g.line_start[lx0 + dl + 1]) ++dl;
+    cof[i] = ZB * dl * LD + PD::idx(g.col_y[c]);
+  }
   __syncthreads();
   cplx<T>* wbuf = buf + wave * LPW * LD;
   for (int ll = 0; ll < LPW; ++ll) {
@@ -309,17 +319,12 @@ k_ypass_fwd(GridDev g, const cplx<T>* __restrict__ t2, cplx<T>* __restrict__ t1)
   wave_sync();
   line_fft<-1, EPL, N1, N2, LPW>(wbuf, LPW, LD, g.ax[1], tw, lane);
   __syncthreads();
-  const int c0 = g.line_start[lx0], c1 = g.line_start[lx1];
   const int nwork = (c1 - c0) * ZB;
   for (int f = threadIdx.x; f < nwork; f += NTHR) {
     const int dz = f % ZB;
     const int c = c0 + f / ZB;
     const int z = z0 + dz;
-    if (z < np2) {
-      int dl = 0;
-      while (lx0 + dl + 1 < lx1 && c >= g.line_start[lx0 + dl + 1]) ++dl;
-      t1[((long)slot * g.ncol + c) * np2 + z] = buf[(dz + ZB * dl) * LD + PD::idx(g.col_y[c])];
-    }
+    if (z < np2) t1[((long)slot * g.ncol + c) * np2 + z] = buf[dz * LD + cof[c - c0]];
   }
 }
 
@@ -927,7 +932,9 @@ static size_t lds_z(const GridDev& g) {
 }
 template <typename T>
 static size_t lds_y(const GridDev& g) {
-  return sizeof(cplx<T>) * ((size_t)g.np1 + (size_t)Cfg<T>::ZB * Cfg<T>::LB * lds_row(g.np1));
+  // twiddles, ZB*LB lines, and (forward pass) one int per column of LB lines
+  return sizeof(cplx<T>) * ((size_t)g.np1 + (size_t)Cfg<T>::ZB * Cfg<T>::LB * lds_row(g.np1)) +
+         sizeof(int) * (size_t)Cfg<T>::LB * g.np1;
 }
 
 static int check_launch(Ctx* ctx, const char* what) {
