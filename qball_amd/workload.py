@@ -102,6 +102,48 @@ def make_system(ncell=2, ecut_ry=25.0, nbands=None, seed=0, complex_basis=False,
     return s
 
 
+def make_fcc_al(ncell=5, alat0=7.65, ecut_ry=30.0, nbands=900, noccupied=750, seed=0,
+                generate_psi=True):
+    """fcc Al with 4*ncell^3 atoms (conventional cells of alat0 bohr), Γ, ecut_ry, s + p + d
+    projectors (9 per atom: the d form factor g^2 exp(-0.45 g^2) with the l = 2 angular
+    tables of NonLocalPotential::update_twnl, NonLocalPotential.cc:712-747), noccupied
+    doubly occupied bands and the rest empty.  The large-cell form: 500 atoms, 900 bands."""
+    s = System()
+    alat = alat0 * ncell
+    s.a = np.diag([alat, alat, alat])
+    s.ecut = ecut_ry * RY
+    s.omega, s.b = cell_vectors(s.a)
+    s.real = True
+    s.rods = capi.rods_build(s.a, s.ecut, real=True)
+    s.np = density_grid_np(s.a, s.ecut)
+    s.idx, s.kpg = plane_waves(s.rods, s.b)
+    s.ngw = s.idx.shape[0]
+    s.g2 = (s.kpg ** 2).sum(axis=1)
+    base = np.array([[0, 0, 0], [0, .5, .5], [.5, 0, .5], [.5, .5, 0]], dtype=np.float64)
+    s.tau = np.concatenate([(base + np.array([i, j, k], dtype=np.float64)) * alat0
+                            for i in range(ncell) for j in range(ncell) for k in range(ncell)])
+    s.na = s.tau.shape[0]
+    s.nst = nbands
+    s.occ = np.where(np.arange(nbands) < noccupied, 2.0, 0.0)
+    s.seed = seed
+    rng = np.random.default_rng(seed)
+    _fill_tables(s, rng, alat)
+    g = np.sqrt(s.g2)
+    gi = np.where(g > 0, 1.0 / np.where(g > 0, g, 1.0), 0.0)
+    x, y, z = (s.kpg[:, d] * gi for d in range(3))
+    vd = 0.7 * g * g * np.exp(-0.45 * g * g)
+    s15 = math.sqrt(15.0 / (4 * math.pi))
+    s5 = math.sqrt(5.0 / (16 * math.pi))
+    dtab = [s15 * x * y * vd, s15 * y * z * vd, s15 * z * x * vd,
+            0.5 * s15 * (x * x - y * y) * vd, s5 * (3 * z * z - 1.0) * vd]
+    s.twnl = np.concatenate([s.twnl, np.stack(dtab)])
+    s.lproj = np.array(list(s.lproj) + [2] * 5, dtype=np.int32)
+    s.wt = np.array(list(s.wt) + [0.8] * 5)
+    if generate_psi:
+        s.psi = random_psi(s.ngw, s.nst, s.g2, rng, real=True)
+    return s
+
+
 def _fill_tables(s, rng, box):
     """s + p projector tables (4 per atom) and the smooth random v_loc of a cubic box."""
     g = np.sqrt(s.g2)
