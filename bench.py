@@ -46,6 +46,7 @@ def parse():
     p.add_argument("--no-fp32", action="store_true")
     p.add_argument("--no-batch", action="store_true", help="skip the 12-cell batch section")
     p.add_argument("--batch-reps", type=int, default=10)
+    p.add_argument("--no-large", action="store_true", help="skip the Al500 large-cell section")
     p.add_argument("--batch-streams", type=int, default=4,
                    help="contexts (HIP streams) the 12 batch cells are spread over")
     p.add_argument("--no-tune", action="store_true",
@@ -267,6 +268,10 @@ def main():
     if not args.no_batch:
         batch = batch_bench(args)
         log("batch section done")
+    large = None
+    if not args.no_large:
+        large = large_cell_bench(args)
+        log("large-cell section done")
 
     # CPU baseline: the oracle (numpy restatement) on a bounded sample, 1 thread
     cpu = None
@@ -294,6 +299,7 @@ def main():
         "cpu_baseline": cpu,
         "fp32_bulk": fp32,
         "batching": batch,
+        "large_cell": large,
         "tuning": sched64,
         "kernels_sequential_builtin": per_kernel,
         "band_energy_Ha": e64,
@@ -403,6 +409,52 @@ def batch_bench(args):
         c["basis"].close()
     for ctx in ctxs:
         ctx.close()
+    return out
+
+
+def large_cell_bench(args):
+    """fcc Al500 Γ (a = 38.25 bohr, ecut 30 Ry: 77666 plane waves, 900 bands, 144^3 grid,
+    500 x 9 s+p+d projectors): FP64 and FP32-bulk Hψ over all bands with the built-in
+    launch choices, then the FP64 subspace diagonalisation of the FP32-bulk result (the
+    large-cell form: FP32-bulk Hψ + FP64 subspace)."""
+    from qball_amd import QBH_ALL, QBH_FP32, Basis, Context, Hamiltonian
+    from qball_amd.devmem import DeviceArray, synchronize
+    from qball_amd.workload import make_fcc_al
+    s = make_fcc_al(seed=1)
+    ctx = Context(0, None)
+    r = s.rods
+    basis = Basis(ctx, True, r["rod_h"], r["rod_k"], r["rod_lmin"], r["rod_size"], s.np, s.b)
+    ham = Hamiltonian(ctx, basis)
+    v = DeviceArray.from_numpy(s.v)
+    ham.set_vloc(v.ptr)
+    ham.set_projectors([s.na], [len(s.lproj)], s.lproj, s.wt, s.twnl.ravel(), s.tau.ravel(),
+                       s.omega)
+    psi = DeviceArray.from_numpy(np.ascontiguousarray(s.psi.T).view(np.float64).ravel())
+    del s.psi
+    basis.gram(psi.ptr, s.ngw, s.nst)
+    hpsi = DeviceArray(psi.nbytes)
+    out = {"workload": f"fcc Al{s.na} Γ, a=38.25 bohr, ecut 30 Ry, {s.ngw} PWs, {s.nst} bands, "
+                       f"grid {s.np[0]}^3, {s.na * len(s.lproj)} KB projectors (s+p+d)"}
+    for name, fl in (("fp64", QBH_ALL), ("fp32_bulk", QBH_ALL | QBH_FP32)):
+        for _ in range(2):
+            ham.hpsi(psi.ptr, s.ngw, s.nst, s.occ, hpsi.ptr, s.ngw, flags=fl)
+        synchronize()
+        t = time.perf_counter()
+        for _ in range(args.steps):
+            ham.hpsi(psi.ptr, s.ngw, s.nst, s.occ, hpsi.ptr, s.ngw, flags=fl)
+        synchronize()
+        dt = (time.perf_counter() - t) / args.steps
+        out[name] = {"ms_per_hpsi": dt * 1e3, "band_applications_per_s": s.nst / dt}
+    # FP64 subspace diagonalisation of the FP32-bulk Hψ (eigenvalues only; psi unchanged)
+    synchronize()
+    t = time.perf_counter()
+    eig = basis.subspace_diag(psi.ptr, s.ngw, hpsi.ptr, s.ngw, s.nst, eigvec=False)
+    synchronize()
+    out["subspace_diag_fp64_ms"] = (time.perf_counter() - t) * 1e3
+    out["eig_min_max_Ha"] = [float(eig.min()), float(eig.max())]
+    ham.close()
+    basis.close()
+    ctx.close()
     return out
 
 
