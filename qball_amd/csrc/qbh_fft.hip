@@ -379,16 +379,24 @@ k_xpass(GridDev g, cplx<T>* __restrict__ t2, int nslots, int slots_per_wg,
     for (int k = 0; k < EPL; ++k) acc[k] = 0.0;
   }
 
-  // sphere-line element q = lane + k*64: row dy = q % LPW, line lx = q / LPW
+  // sphere-line element q = lane + k*64: row dy = q % LPW, line lx = q / LPW.  Its LDS
+  // offset (-1 past the lines) and its T2 offset within a slot do not depend on the slot:
+  // computed once, so the slot loop issues no index-table loads.
+  const long sstride = (long)np2 * nlx * np1;
+  int lofs[GPL], gofs[GPL];
+#pragma unroll
+  for (int k = 0; k < GPL; ++k) {
+    const int q = lane + k * 64;
+    const int dy = q % LPW, lx = q / LPW;
+    lofs[k] = q < ng ? dy * np0 + g.line_x[lx] : -1;
+    gofs[k] = (q < ng && dy < nyl) ? (z * nlx + lx) * np1 + y0 + dy : -1;
+  }
   cplx<T> pf[GPL];
   auto prefetch = [&](int s) {
+    const cplx<T>* src = t2 + s * sstride;
 #pragma unroll
-    for (int k = 0; k < GPL; ++k) {
-      const int q = lane + k * 64;
-      const int dy = q % LPW, lx = q / LPW;
-      pf[k] = (q < ng && dy < nyl) ? t2[(((long)s * np2 + z) * nlx + lx) * np1 + y0 + dy]
-                                   : cplx<T>{T(0), T(0)};
-    }
+    for (int k = 0; k < GPL; ++k)
+      pf[k] = gofs[k] >= 0 ? src[gofs[k]] : cplx<T>{T(0), T(0)};
   };
   const bool sparse_in = !(OP == X_LOAD || OP == X_LOAD_REAL);
   if (sparse_in && s0 < s1) prefetch(s0);
@@ -398,10 +406,8 @@ k_xpass(GridDev g, cplx<T>* __restrict__ t2, int nslots, int slots_per_wg,
       for (int i = lane; i < LPW * np0; i += 64) buf[i] = {T(0), T(0)};
       wave_sync();
 #pragma unroll
-      for (int k = 0; k < GPL; ++k) {
-        const int q = lane + k * 64;
-        if (q < ng) buf[(q % LPW) * np0 + g.line_x[q / LPW]] = pf[k];
-      }
+      for (int k = 0; k < GPL; ++k)
+        if (lofs[k] >= 0) buf[lofs[k]] = pf[k];
       wave_sync();
       if (s + 1 < s1) prefetch(s + 1);
       fft_lines_w<+1, EPL>(buf, LPW, np0, g.ax[0], tw, lane);
@@ -477,10 +483,11 @@ k_xpass(GridDev g, cplx<T>* __restrict__ t2, int nslots, int slots_per_wg,
 
     // VMUL / LOAD*: forward x transform and store of the sphere lines
     fft_lines_w<-1, EPL>(buf, LPW, np0, g.ax[0], tw, lane);
-    for (int q = lane; q < ng; q += 64) {
-      const int dy = q % LPW, lx = q / LPW;
-      if (dy < nyl)
-        t2[(((long)s * np2 + z) * nlx + lx) * np1 + y0 + dy] = buf[dy * np0 + g.line_x[lx]];
+    {
+      cplx<T>* dst = t2 + s * sstride;
+#pragma unroll
+      for (int k = 0; k < GPL; ++k)
+        if (gofs[k] >= 0) dst[gofs[k]] = buf[lofs[k]];
     }
     wave_sync();
   }

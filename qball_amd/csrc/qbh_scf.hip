@@ -90,7 +90,9 @@ k_psda_ab(int ngw, int nst, int real, const double* __restrict__ occ,
   }
 }
 
-// c = x + th (x - xl) + f + th (f - fl); xl = x; fl = f  (th = 0: c = x + f)
+// c = x + th (x - xl) + f + th (f - fl); xl = x; fl = f.  th = 0 (the first step, or no
+// extrapolation): c = x + f without reading xl, fl (the same value for finite history, and
+// no 0 * NaN from history that was never written)
 __global__ void k_psda_step(int ngw, int nst, double th, double* __restrict__ c, long ld,
                             const double* __restrict__ dc, long ldd, double* __restrict__ cl,
                             double* __restrict__ dcl) {
@@ -98,7 +100,7 @@ __global__ void k_psda_step(int ngw, int nst, double th, double* __restrict__ c,
   if (i >= 2L * ngw * nst) return;
   const long n = i / (2L * ngw), r = i - n * 2L * ngw;
   const double x = c[n * 2 * ld + r], f = dc[n * 2 * ldd + r];
-  c[n * 2 * ld + r] = x + th * (x - cl[i]) + f + th * (f - dcl[i]);
+  c[n * 2 * ld + r] = th != 0.0 ? x + th * (x - cl[i]) + f + th * (f - dcl[i]) : x + f;
   cl[i] = x;
   dcl[i] = f;
 }
@@ -290,6 +292,9 @@ extern "C" int qbh_psda_create(qbh_basis basis, int nst, double ecutprec, qbh_ps
     return sfail(c, QBH_ERR_ALLOC, "PSDA buffers allocation failed");
   }
   k_prec_table<<<g256(ngw), 256, 0, c->stream>>>(ngw, p->b->kpg2, ecutprec, p->K);
+  // the extrapolation history starts cleared (written before it is first read)
+  k_zero<<<g256((long)blk), 256, 0, c->stream>>>(p->cl, (long)blk);
+  k_zero<<<g256((long)blk), 256, 0, c->stream>>>(p->dcl, (long)blk);
   *out = p;
   return scheck(c, "psda create");
 }
