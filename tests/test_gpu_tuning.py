@@ -200,3 +200,49 @@ def test_cleared_output_is_never_read(gpu_ctx, flags, tol):
             assert rel_err(a, _ref(c)) <= tol, (flags, algo)
     ham.close()
     c["basis"].close()
+
+
+@pytest.mark.parametrize("which", ["si8", "si8c"])
+def test_small_workspace_chunks_give_identical_results(which):
+    """A transform workspace that holds only two FP64 slots splits every pass into many
+    chunks (and changes the slots per x-pass workgroup); each slot's arithmetic is
+    unchanged, so Hψ (FP64 and FP32 form) and the transforms are bitwise the same as with
+    one chunk, and the density (summed over chunks) equal to rounding."""
+    import torch
+    from qball_amd import Context
+    ctx = Context(0, torch.cuda.current_stream().cuda_stream)
+    c = _cell(ctx, which, None, seed=8)
+    s = c["sysm"]
+    basis = c["basis"]
+    ngw = basis.ngw
+    n0, n1, n2 = (int(x) for x in s.np)
+    N = n0 * n1 * n2
+    cd = to_dev(s.psi)
+
+    def run_all():
+        out = []
+        for fl in (0x7, 0x17):
+            hd = torch.zeros_like(cd)
+            c["ham"].hpsi(cd.data_ptr(), ngw, s.nst, s.occ, hd.data_ptr(), ngw, flags=fl)
+            out.append(hd)
+        rho = torch.zeros(N, dtype=torch.float64, device="cuda")
+        basis.density(cd.data_ptr(), ngw, s.nst, s.occ, 1.0, s.omega, rho.data_ptr())
+        f = torch.zeros(2 * N * s.nst, dtype=torch.float64, device="cuda")
+        basis.backward(cd.data_ptr(), ngw, s.nst, 0, f.data_ptr())
+        back = torch.zeros_like(cd)
+        basis.forward(f.data_ptr(), s.nst, 0, back.data_ptr(), ngw)
+        ctx.sync()
+        return out + [rho, f, back]
+
+    ref = run_all()
+    slot = 16 * (basis.ncol * n2 + n2 * basis.nlx * n1)   # FP64 T1 + T2 of one slot
+    ctx.set_workspace_cap(2 * slot + 1)
+    got = run_all()
+    for k, (a, b) in enumerate(zip(got, ref)):
+        if k == 2:  # the density sums the chunks in a different grouping
+            assert float((a - b).abs().max()) <= 1e-14 * float(b.abs().max()), k
+        else:
+            assert torch.equal(a, b), k
+    c["ham"].close()
+    basis.close()
+    ctx.close()
