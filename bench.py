@@ -219,6 +219,33 @@ def main():
         roof = {"kernel": dom, "bound": "hbm", "achieved": d.get("GBps"), "peak": PEAK_HBM_GBS,
                 "unit": "GB/s", "frac": (d.get("GBps") or 0) / PEAK_HBM_GBS}
     roof["traffic"] = d.get("hbm_bytes_per_launch")
+    # FFT stages (the five transform passes) as one: time per Hψ, the bandwidth on this
+    # pipeline's algorithmic bytes, on the counted DRAM bytes, and the "equivalent" one on
+    # the unfused pruned-pencil model of qball's FourierTransform (per Γ pair:
+    # 2(2 nvec np2 s) z + 2(4 ntrans0 np0 np2 s) pruned x + 2(2 N s) y + 2(nvec np2 + N) s
+    # zvec<->grid + (2 N s + N r) xv + 6 ngw s coefficients in / Hψ RMW)
+    fft_names = ("zpass_bwd", "ypass_bwd", "xpass_vmul", "ypass_fwd", "zpass_fwd")
+    t_fft = sum(per_kernel[k]["avg_ms"] * per_kernel[k]["launches_per_step"] for k in fft_names)
+    b_fft = sum(per_kernel[k]["bytes_per_launch"] * per_kernel[k]["launches_per_step"]
+                for k in fft_names)
+    b_dram = (sum(traffic[k] * per_kernel[k]["launches_per_step"] for k in fft_names)
+              if all(k in traffic for k in fft_names) else None)
+    n0, n1, n2 = sysm.np
+    N = n0 * n1 * n2
+    nvec, ntrans0 = sysm.ncol, (sysm.nlx + 1) // 2
+    b_pair = (2 * (2 * nvec * n2 * 16) + 2 * (4 * ntrans0 * n0 * n2 * 16) + 2 * (2 * N * 16)
+              + 2 * (nvec * n2 + N) * 16 + (2 * N * 16 + N * 8) + 6 * ngw * 16)
+    b_model = b_pair * nslots_total
+    fft = {"ms_per_hpsi": t_fft,
+           "algorithmic_bytes": b_fft, "GBps": b_fft / t_fft / 1e6,
+           "frac_of_peak": b_fft / t_fft / 1e6 / PEAK_HBM_GBS,
+           "dram_bytes": b_dram,
+           "dram_GBps": (b_dram / t_fft / 1e6) if b_dram else None,
+           "pruned_pencil_model_bytes": b_model,
+           "equivalent_GBps": b_model / t_fft / 1e6,
+           "equivalent_frac_of_peak": b_model / t_fft / 1e6 / PEAK_HBM_GBS,
+           "note": "five transform passes of one FP64 Hψ, sequential characterisation pass; "
+                   "dram_bytes from the committed FETCH/WRITE counter runs"}
     roof["measured_in"] = ("sequential characterisation pass at the built-in launch choices, "
                            f"{char_steps} steps, HIP events on the library stream")
 
@@ -296,6 +323,7 @@ def main():
                                f"{ngw} PWs, {nst} bands, grid {sysm.np[0]}^3, {4 * sysm.na} KB projectors",
                    "global_batch": nst, "parallelism": "single GPU"},
         "roofline": roof,
+        "fft_stages": fft,
         "cpu_baseline": cpu,
         "fp32_bulk": fp32,
         "batching": batch,
