@@ -54,6 +54,10 @@ template <typename T> struct Cfg;
 #ifndef QBH_XWPE32
 #define QBH_XWPE32 3
 #endif
+// x pass: stage-B forward twiddles held in registers across slots (1) or formed per slot
+#ifndef QBH_XTWB
+#define QBH_XTWB 1
+#endif
 // x pass: padded LDS rows (element x at x + x/N2).  The x pass holds two workgroups per CU
 // (VGPR-bound), so the larger rows cost it no occupancy, unlike the y/z passes.
 #ifndef QBH_XLDS_PAD
@@ -545,6 +549,27 @@ k_xpass2(GridDev g, cplx<T>* __restrict__ t2, int nslots, int slots_per_wg,
 
   T vr[IB][N2];
   double acc[IB][N2];
+  // stage-B forward twiddles W_n^{-k1 m2}, m2 = 1..N2-1, of this lane's k1: the same
+  // repeated products twiddle_row forms, computed once instead of per slot (measured at
+  // 140 = 14*10: x pass 6.24 -> 6.01 ms per FP64 Hψ).  Not for the FP64 kernels with the
+  // full line-element count, which have no registers to spare.
+  constexpr int GS = (YB * (n / 2 + 8) + NTHR - 1) / NTHR;
+  constexpr bool TWB = QBH_XTWB && (sizeof(T) == 4 || GPW <= GS);
+  cplx<T> twb[IB][TWB ? N2 - 1 : 1];
+  if constexpr (TWB && OP == X_VMUL) {
+#pragma unroll
+    for (int ib = 0; ib < IB; ++ib) {
+      const int k1 = (lane + 64 * ib) % N1;
+      cplx<T> w = tw[k1];
+      w.y = -w.y;
+      cplx<T> p = w;
+#pragma unroll
+      for (int m = 1; m < N2; ++m) {
+        twb[ib][m - 1] = p;
+        if (m + 1 < N2) p = cmul(p, w);
+      }
+    }
+  }
 #pragma unroll
   for (int ib = 0; ib < IB; ++ib) {
     const int t = lane + 64 * ib;
@@ -677,7 +702,14 @@ k_xpass2(GridDev g, cplx<T>* __restrict__ t2, int nslots, int slots_per_wg,
 #pragma unroll
           for (int k2 = 0; k2 < N2; ++k2) b[k2] = {vr[ib][k2] * b[k2].x, vr[ib][k2] * b[k2].y};
           sdft<N2, -1>(b);
-          twiddle_row<N2, -1>(b, tw, k1);
+          if constexpr (TWB) {
+            if (k1 != 0) {
+#pragma unroll
+              for (int m = 1; m < N2; ++m) b[m] = cmul(b[m], twb[ib][m - 1]);
+            }
+          } else {
+            twiddle_row<N2, -1>(b, tw, k1);
+          }
 #pragma unroll
           for (int m2 = 0; m2 < N2; ++m2) blk[m2] = b[m2];
         } else {

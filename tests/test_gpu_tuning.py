@@ -246,3 +246,36 @@ def test_small_workspace_chunks_give_identical_results(which):
     c["ham"].close()
     basis.close()
     ctx.close()
+
+
+@pytest.mark.parametrize("flags,tol", [(0x7, TOL), (0x17, 2e-6)])
+def test_wide_sphere_uses_full_line_count(gpu_ctx, flags, tol):
+    """A sphere spanning most of the x edge (8-atom Si at 64 Ry on a 36^3 grid: 27 of 36 x
+    positions hold lines) takes the x-pass variant with the full line-element count per
+    thread; Hψ still matches the oracle."""
+    import torch
+    from oracle import hpsi as oh
+    from oracle.basis import FFTMaps
+    from qball_amd import Basis, Hamiltonian
+    from qball_amd.workload import make_system
+    s = make_system(ncell=1, ecut_ry=64.0, nbands=6, seed=9)
+    grid = (36, 36, 36)
+    ob, _, sp = oracle_for(s)
+    maps = FFTMaps(ob, *grid)
+    r = s.rods
+    basis = Basis(gpu_ctx, True, r["rod_h"], r["rod_k"], r["rod_lmin"], r["rod_size"], grid, s.b)
+    assert basis.nlx > 36 // 2 + 8, basis.nlx
+    ham = Hamiltonian(gpu_ctx, basis)
+    v = np.random.default_rng(19).standard_normal(maps.N) * 0.3
+    vd = real_to_dev(v)
+    ham.set_vloc(vd.data_ptr())
+    ham.set_projectors([s.na], [len(s.lproj)], s.lproj, s.wt, s.twnl.ravel(), s.tau.ravel(),
+                       s.omega)
+    cd = to_dev(s.psi)
+    hd = torch.zeros_like(cd)
+    ham.hpsi(cd.data_ptr(), basis.ngw, s.nst, s.occ, hd.data_ptr(), basis.ngw, flags=flags)
+    gpu_ctx.sync()
+    ref = oh.hpsi(maps, s.psi, v, [sp], s.occ, s.omega)[0]
+    assert rel_err(from_dev(hd, basis.ngw, s.nst), ref) <= tol
+    ham.close()
+    basis.close()
