@@ -279,6 +279,24 @@ def main():
         dl1 = float(np.abs(dens["fp32"] - dens["fp64"]).sum() * sysm.omega / N)
         ne = float(dens["fp64"].sum() * sysm.omega / N)
         del dens
+        # non-local ionic forces (the force term the projector products feed), FP32 form
+        # against FP64 over all bands; one untimed call each, then one timed
+        forces = {}
+        for name, f32 in (("fp64", False), ("fp32", True)):
+            ham.nl_forces(psi.ptr, ngw, nst, sysm.occ, fp32=f32)
+            t0 = time.perf_counter()
+            forces[name] = ham.nl_forces(psi.ptr, ngw, nst, sysm.occ, fp32=f32)
+            forces[name + "_ms"] = (time.perf_counter() - t0) * 1e3
+        fdiff = forces["fp32"] - forces["fp64"]
+        force_err = {
+            "max_abs_err_Ha_per_bohr": float(np.max(np.abs(fdiff))),
+            "rel_err": float(np.linalg.norm(fdiff) / np.linalg.norm(forces["fp64"])),
+            "max_abs_force_Ha_per_bohr": float(np.max(np.abs(forces["fp64"]))),
+            "call_ms": {"fp64": forces["fp64_ms"], "fp32": forces["fp32_ms"]},
+            "note": "non-local ionic forces (qbh_nl_forces, all atoms and bands); this path "
+                    "computes no local or ionic force terms"}
+        log(f"nl forces fp32 vs fp64: max {force_err['max_abs_err_Ha_per_bohr']:.3e} Ha/bohr, "
+            f"rel {force_err['rel_err']:.3e}")
         apply(builtin)
         _, k32c = timed(QBH_ALL | QBH_FP32, char_steps)
         k32 = {k: {"avg_ms": ms / cnt, "launches_per_step": cnt / char_steps}
@@ -287,7 +305,7 @@ def main():
                 "hpsi_rel_err_vs_fp64": rel,
                 "band_energy_diff_per_atom_Ha": abs(e32 - e64) / sysm.na,
                 "density_L1_vs_fp64_electrons": dl1, "density_L1_rel": dl1 / ne,
-                "nelectrons": ne,
+                "nelectrons": ne, "nl_force_error": force_err,
                 "tuning": sched32, "kernels_sequential_builtin": k32}
     del h64
 

@@ -129,6 +129,16 @@ int qbh_ham_set_vector_potential(qbh_ham ham, const double* A);
 int qbh_hpsi(qbh_ham ham, const double* psi_dev, long ld, int nst, const double* occ,
              double* hpsi_dev, long ldh, int flags, double* enl);
 
+/* Non-local ionic forces (NonLocalPotential::energy(..., compute_forces=true, fion, ...),
+ * norm-conserving branch, NonLocalPotential.cc:2175-2323; fion[is][3*ia+j] = tmpfion at
+ * :2617-2622): fion (host, 3 per atom, atoms in the species order of set_projectors) is
+ * overwritten with -dEnl/dtau = -sum_n 2 occ_n (wt/Omega) Re(fnl conj(dfnl_j)) (Γ: x2 for
+ * G, -G in dfnl), dfnl_j the contraction with the (k+G-A)_j (-i)^(l+1) table.  enl
+ * (nullable, host) receives Enl.  flags: QBH_FP32 takes both projector products in FP32
+ * (FP64 sums); 0 = FP64. */
+int qbh_nl_forces(qbh_ham ham, const double* psi_dev, long ld, int nst, const double* occ,
+                  double* fion, int flags, double* enl);
+
 /* ρ(r) += (weight/omega) * occ[n] * |psi_n(r)|^2 over bands with positive factor
  * (SlaterDet::compute_density, SlaterDet.cc:839-932).  rho_dev is np0*np1*np2 doubles;
  * the caller zeroes it (ChargeDensity::update_density, ChargeDensity.cc:287-290). */

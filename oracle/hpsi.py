@@ -157,6 +157,95 @@ def nonlocal_energy(basis, species, c, occ, omega, cp=None, kpgx=None, na_block=
     return enl
 
 
+def danl_block(sp, kpgx, j, ia0, ia1):
+    """The force table of NonLocalPotential.cc:2188-2236 for direction j and atoms
+    [ia0, ia1): column ipra = ia + ipr*nblk holds (k+G)_j twnl (-i)^(l+1) exp(-i (k+G).tau),
+    written with the per-l real/imaginary pattern of :2204-2233 from cos/sin of
+    kpgr = -(k+G).tau (comp_eigr, :1966-1975)."""
+    tau = sp.tau[ia0:ia1]
+    nblk = ia1 - ia0
+    kpgr = -(kpgx[:, 0:1] * tau[None, :, 0] + kpgx[:, 1:2] * tau[None, :, 1]
+             + kpgx[:, 2:3] * tau[None, :, 2])
+    cs = np.cos(kpgr)
+    sn = np.sin(kpgr)
+    danl = np.empty((kpgx.shape[0], sp.npr * nblk), dtype=np.complex128)
+    for ipr in range(sp.npr):
+        tt = (kpgx[:, j] * sp.twnl[ipr])[:, None]
+        l = int(sp.lproj[ipr])
+        if l == 0:
+            d = tt * sn + 1j * (-tt * cs)
+        elif l == 1:
+            d = -tt * cs + 1j * (-tt * sn)
+        elif l == 2:
+            d = -tt * sn + 1j * (tt * cs)
+        elif l == 3:
+            d = tt * cs + 1j * (tt * sn)
+        else:
+            raise ValueError("l > 3")
+        danl[:, ipr * nblk:(ipr + 1) * nblk] = d
+    return danl
+
+
+def nonlocal_forces(basis, species, c, occ, omega, kpgx=None, na_block=None):
+    """Non-local ionic forces, NonLocalPotential::energy(compute_forces=true),
+    norm-conserving branch (NonLocalPotential.cc:2175-2323; fnl scaled by wt/Omega at
+    :2106-2148 first; fion[is][3*ia+j] = tmpfion at :2617-2622).
+
+    For each direction j the table anl_j = (k+G)_j twnl (-i)^(l+1) exp(-i (k+G).tau)
+    (:2199-2234) gives dfnl = 2 anl_j^T c over the real view at Γ (no G=0 term, :2262-2267)
+    or anl_j^H c (:2254-2259), and fion[ia, j] -= 2 occ_n fnl dfnl (Γ, :2283-2288) or
+    2 occ_n Re(fnl conj(dfnl)) (:2307-2311), fnl already multiplied by wt/Omega.
+    ``kpgx`` may carry -A (VectorPotential::get_kpgpax, :2187).  Returns (enl, fion) with
+    fion a list of (na, 3) arrays, one per species."""
+    if kpgx is None:
+        kpgx = basis.kpgx
+    omega_inv = 1.0 / omega
+    namax = max(sp.na for sp in species)
+    if na_block is None:
+        na_block = 128 if namax > 128 else namax     # :1535-1541
+    enl = 0.0
+    fion = []
+    for sp in species:
+        f_sp = np.zeros((sp.na, 3))
+        fion.append(f_sp)
+        if sp.npr == 0:
+            continue
+        nblocks = (sp.na + na_block - 1) // na_block
+        for ib in range(nblocks):
+            ia0 = ib * na_block
+            ia1 = min(sp.na, ia0 + na_block)
+            nblk = ia1 - ia0
+            anl = anl_block(sp, kpgx, ia0, ia1)
+            if basis.real:
+                fnl = anl.real.T @ c.real + anl.imag.T @ c.imag
+                fnl = fnl - 0.5 * np.outer(anl[0].real, c[0].real)
+                fnl = 2.0 * fnl
+            else:
+                fnl = anl.conj().T @ c
+            for ipr in range(sp.npr):
+                fac = sp.wt[ipr] * omega_inv
+                rows = slice(ipr * nblk, (ipr + 1) * nblk)
+                t = fnl[rows]
+                if basis.real:
+                    enl += float(np.sum((fac * occ)[None, :] * t * t))
+                else:
+                    enl += float(np.sum((fac * occ)[None, :] * (t.real ** 2 + t.imag ** 2)))
+                fnl[rows] = fac * t
+            for j in range(3):
+                danl = danl_block(sp, kpgx, j, ia0, ia1)
+                if basis.real:
+                    dfnl = 2.0 * (danl.real.T @ c.real + danl.imag.T @ c.imag)
+                    prod = fnl * dfnl
+                else:
+                    dfnl = danl.conj().T @ c
+                    prod = (fnl * dfnl.conj()).real
+                facn = 2.0 * np.asarray(occ, dtype=np.float64)
+                contrib = (prod * facn[None, :]).sum(axis=1)      # per column ipra
+                for ipr in range(sp.npr):
+                    f_sp[ia0:ia1, j] -= contrib[ipr * nblk:(ipr + 1) * nblk]
+    return enl, fion
+
+
 # --------------------------------------------------------------------------------------
 def hpsi(maps, c, v, species, occ, omega, kpg2=None, kpgx=None):
     """EnergyFunctional::energy(compute_hpsi=true) Hψ part (EnergyFunctional.cc:1142-1153

@@ -49,6 +49,7 @@ SIGNATURES = {
     "qbh_ham_set_projectors": (_c_int, [_vp, _c_int, _ip, _ip, _ip, _dp, _dp, _dp, _c_double]),
     "qbh_ham_set_vector_potential": (_c_int, [_vp, _dp]),
     "qbh_hpsi": (_c_int, [_vp, _vp, _c_long, _c_int, _dp, _vp, _c_long, _c_int, _dp]),
+    "qbh_nl_forces": (_c_int, [_vp, _vp, _c_long, _c_int, _dp, _dp, _c_int, _dp]),
     "qbh_density": (_c_int, [_vp, _vp, _c_long, _c_int, _dp, _c_double, _c_double, _vp, _c_int]),
     "qbh_gram": (_c_int, [_vp, _vp, _c_long, _c_int]),
     "qbh_subspace_diag": (_c_int, [_vp, _vp, _c_long, _vp, _c_long, _c_int, _dp, _c_int]),
@@ -275,6 +276,7 @@ class Hamiltonian:
     def __init__(self, ctx, basis):
         self.ctx, self.basis = ctx, basis
         self.h = _vp()
+        self._natoms = 0
         ctx.check(lib().qbh_ham_create(ctx.h, basis.h, ctypes.byref(self.h)), "qbh_ham_create")
 
     def set_vloc(self, v_ptr):
@@ -282,6 +284,7 @@ class Hamiltonian:
 
     def set_projectors(self, na, npr, lproj, wt, twnl, tau, omega):
         keep = [_ia(na), _ia(npr), _ia(lproj), _da(wt), _da(twnl), _da(tau)]
+        self._natoms = int(np.sum(keep[0][0]))
         self.ctx.check(lib().qbh_ham_set_projectors(self.h, len(keep[0][0]),
                                                     *[k[1] for k in keep], float(omega)),
                        "qbh_ham_set_projectors")
@@ -296,6 +299,20 @@ class Hamiltonian:
         self.ctx.check(lib().qbh_hpsi(self.h, psi_ptr, ld, nst, op, hpsi_ptr, ldh, flags,
                                       ctypes.byref(enl) if want_enl else None), "qbh_hpsi")
         return enl.value if want_enl else None
+
+    def nl_forces(self, psi_ptr, ld, nst, occ, fp32=False, want_enl=False):
+        """Non-local ionic forces (natoms, 3), atoms in species order; with want_enl also
+        Enl: (fion, enl)."""
+        o_, op = _da(occ)
+        natoms = int(self._natoms)
+        f = np.zeros(3 * max(natoms, 1))
+        enl = ctypes.c_double(0.0)
+        self.ctx.check(lib().qbh_nl_forces(self.h, psi_ptr, ld, nst, op, f.ctypes.data_as(_dp),
+                                           QBH_FP32 if fp32 else 0,
+                                           ctypes.byref(enl) if want_enl else None),
+                       "qbh_nl_forces")
+        f = f[:3 * natoms].reshape(natoms, 3)
+        return (f, enl.value) if want_enl else f
 
     def scf_run(self, vbasis, psi_ptr, ld, nst, occ, omega, vion_g_ptr, rhops_g_ptr, nitscf,
                 nite, ecutprec, alpha=0.5, ndim=3, rc_kerker=10.0, flags_inner=QBH_ALL,

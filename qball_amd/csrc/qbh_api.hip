@@ -200,6 +200,7 @@ static void free_proj(Ham* h) {
   h->anl64 = h->anl32 = nullptr;
   h->anl64_ok = h->anl32_ok = false;
   h->P = 0;
+  h->col_atom_host.clear();
 }
 
 extern "C" int qbh_ham_destroy(qbh_ham ham) {
@@ -209,7 +210,7 @@ extern "C" int qbh_ham_destroy(qbh_ham ham) {
   hipStreamSynchronize(h->ctx->stream);
   free_proj(h);
   for (void* p : {(void*)h->v, (void*)h->kin, h->f64, h->f32, h->psi32, h->tmp32, (void*)h->red,
-                  h->etrs})
+                  h->etrs, h->dpsi, h->df, h->fpart, h->fg})
     if (p) hipFree(p);
   for (void* p : h->expo) hipFree(p);
   delete ham;
@@ -284,6 +285,7 @@ extern "C" int qbh_ham_set_projectors(qbh_ham ham, int nsp, const int* na, const
     free_proj(h);
     return fail(c, QBH_ERR_ALLOC, "projector table allocation failed");
   }
+  h->col_atom_host = cat;
   return dev_check(c, "set_projectors");
 }
 
@@ -397,6 +399,20 @@ extern "C" int qbh_hpsi(qbh_ham ham, const double* psi_dev, long ld, int nst, co
     return fail(h->ctx, QBH_ERR_ARG, "psi/hpsi must be 16-byte aligned");
   if (nst > 65535) return fail(h->ctx, QBH_ERR_ARG, "nst > 65535");
   return hpsi_impl(h, psi_dev, ld, nst, occ, hpsi_dev, ldh, flags, enl);
+}
+
+extern "C" int qbh_nl_forces(qbh_ham ham, const double* psi_dev, long ld, int nst,
+                             const double* occ, double* fion, int flags, double* enl) {
+  if (!ham || !psi_dev || !fion || nst <= 0) return QBH_ERR_ARG;
+  Ham* h = &ham->h;
+  hipSetDevice(h->ctx->device);
+  if (ld < h->b->g.ngw) return fail(h->ctx, QBH_ERR_ARG, "ld < ngw");
+  if (reinterpret_cast<uintptr_t>(psi_dev) & 15)
+    return fail(h->ctx, QBH_ERR_ARG, "psi must be 16-byte aligned");
+  if (nst > 65535) return fail(h->ctx, QBH_ERR_ARG, "nst > 65535");
+  if (flags & ~QBH_FP32) return fail(h->ctx, QBH_ERR_ARG, "flags: only QBH_FP32 applies");
+  if (int r = nl_forces(h, psi_dev, ld, nst, occ, (flags & QBH_FP32) != 0, fion, enl)) return r;
+  return dev_check(h->ctx, "nl_forces");
 }
 
 // ------------------------------------------------------------------------------------

@@ -22,6 +22,7 @@ struct Ham {
   int* col_tw = nullptr;          // per column: row of the twnl table
   int* col_atom = nullptr;        // per column: atom index into tau
   double* col_fac = nullptr;      // per column: wt/Omega
+  std::vector<int> col_atom_host; // col_atom on the host (force accumulation)
   double* twnl = nullptr;         // concatenated tables, row = species projector
   double* tau = nullptr;          // 3 * total atoms
   void* anl64 = nullptr;          // Γ: 2ngw x P real; complex: ngw x P complex
@@ -36,6 +37,14 @@ struct Ham {
   size_t psi32_bytes = 0;
   void* tmp32 = nullptr;
   size_t tmp32_bytes = 0;
+  void* dpsi = nullptr;           // forces: i (k+G-A)_j psi (FP64)
+  size_t dpsi_bytes = 0;
+  void* df = nullptr;             // forces: dfnl partials
+  size_t df_bytes = 0;
+  void* fpart = nullptr;          // forces: per-column partial sums over band chunks
+  size_t fpart_bytes = 0;
+  void* fg = nullptr;             // forces: per-column sums, 3 x P
+  size_t fg_bytes = 0;
   double* red = nullptr;          // reduction partials, then occupations
   size_t red_n = 0;
   std::vector<double> occ_host;   // occupations currently on the device
@@ -66,6 +75,9 @@ int nl_apply(Ham* h, const double* psi, long ld, int nst, const double* occ, dou
              long ldh, int fp32, double* enl, const float** defer_add32 = nullptr,
              bool overwrite = false);
 double nl_read_enl(Ham* h, hipStream_t s);
+// non-local ionic forces of qbh_nl_forces into fion (host, 3 per atom, overwritten)
+int nl_forces(Ham* h, const double* psi, long ld, int nst, const double* occ, int fp32,
+              double* fion, double* enl);
 // number of hipBLASLt algorithms listed for the fnl (which 0) or enl (which 1) GEMM of an
 // Hψ call with these arguments (0: hipBLASLt does not cover it)
 int nl_lt_candidates(Ham* h, int which, int nst, long ld, long ldh, int fp32);

@@ -460,62 +460,62 @@ int nl_lt_candidates(Ham* h, int which, int nst, long ld, long ldh, int fp32) {
   return p ? (int)p->cand.size() : 0;
 }
 
-int nl_apply(Ham* h, const double* psi, long ld, int nst, const double* occ_host, double* hpsi,
-             long ldh, int fp32, double* enl, const float** defer_add32, bool overwrite) {
+// Occupations on the device (h->red + kRedBlocks + 1, uploaded when they change) after
+// the Enl reduction partials
+constexpr int kRedBlocks = 512;
+static int nl_occ(Ham* h, int nst, const double* occ_host, const double** occ) {
   Ctx* c = h->ctx;
-  if (h->P == 0) {
-    if (enl) *enl = 0.0;
-    return 0;
-  }
-  const int ngw = h->b->g.ngw;
-  const bool real = h->b->g.real;
-  const bool cplx = !real;
-  const int P = h->P;
-  if (int r = ensure_anl(h, fp32)) return r;
-  void* hb;
-  if (int r = blas_handle(c, &hb)) return r;
-  rocblas_handle bh = static_cast<rocblas_handle>(hb);
-
-  // occupations and reduction scratch
-  const int nblk = 512;
-  if (h->red_n < (size_t)(nblk + 1 + nst)) {
+  if (h->red_n < (size_t)(kRedBlocks + 1 + nst)) {
     if (h->red) hipFree(h->red);
     h->red = nullptr;
     h->red_n = 0;
     h->occ_host.clear();
-    if (hipMalloc(&h->red, (nblk + 1 + nst) * sizeof(double)) != hipSuccess) {
+    if (hipMalloc(&h->red, (kRedBlocks + 1 + nst) * sizeof(double)) != hipSuccess) {
       c->last_error = "reduction scratch allocation failed";
       return QBH_ERR_ALLOC;
     }
-    h->red_n = nblk + 1 + nst;
+    h->red_n = kRedBlocks + 1 + nst;
   }
-  double* occ = h->red + nblk + 1;
-  {
-    // occupations change rarely: upload only when they differ from the device copy
-    std::vector<double> o(nst, 0.0);
-    if (occ_host)
-      for (int n = 0; n < nst; ++n) o[n] = occ_host[n];
-    if (o != h->occ_host) {
-      hipMemcpyAsync(occ, o.data(), nst * sizeof(double), hipMemcpyHostToDevice, c->stream);
-      hipStreamSynchronize(c->stream);  // o is a stack buffer
-      h->occ_host = o;
-    }
+  double* d = h->red + kRedBlocks + 1;
+  // occupations change rarely: upload only when they differ from the device copy
+  std::vector<double> o(nst, 0.0);
+  if (occ_host)
+    for (int n = 0; n < nst; ++n) o[n] = occ_host[n];
+  if (o != h->occ_host) {
+    hipMemcpyAsync(d, o.data(), nst * sizeof(double), hipMemcpyHostToDevice, c->stream);
+    hipStreamSynchronize(c->stream);  // o is a stack buffer
+    h->occ_host = o;
   }
+  *occ = d;
+  return 0;
+}
 
+// The projector coefficients of an Hψ call: F = anl^T psi over the real view (Γ) or
+// anl^H psi, finished (K-slice partials summed; Γ: G=0 term and x2), with Enl partials in
+// h->red and F scaled by wt/Omega (NonLocalPotential.cc:2040-2148).  FP32: psi is first
+// converted into h->psi32 and F is FP32 (h->f32), else FP64 (h->f64).
+struct FnlStage {
+  void* F = nullptr;      // P x nst (complex: interleaved) in slot 0 of the partials
+  int sel = 0;            // fnl_algo value in force (0: rocBLAS over K slices)
+};
+
+static int fnl_stage(Ham* h, rocblas_handle bh, const double* psi, long ld, int nst, long ldh,
+                     bool fp32, const double* occ, FnlStage* o) {
+  Ctx* c = h->ctx;
+  const int ngw = h->b->g.ngw;
+  const bool real = h->b->g.real;
+  const bool cplx = !real;
+  const int P = h->P;
   const long Kelem = real ? 2L * ngw : (long)ngw;  // contraction length in elements
   const int e = cplx ? 2 : 1;
-  // hipBLASLt algorithms chosen for this shape (nullptr: rocBLAS)
-  const int fsel = h->fnl_algo >= 0 ? h->fnl_algo : nl_builtin_algo(c);
-  const int esel = h->enl_algo >= 0 ? h->enl_algo : nl_builtin_algo(c);
-  LtPlan* fplan = fsel > 0 ? nl_plan(h, 0, nst, ld, ldh, fp32 != 0) : nullptr;
-  LtPlan* eplan = esel > 0 ? nl_plan(h, 1, nst, ld, ldh, fp32 != 0) : nullptr;
-  const int falgo = fplan ? std::min<int>(fsel, (int)fplan->cand.size()) - 1 : -1;
-  const int ealgo = eplan ? std::min<int>(esel, (int)eplan->cand.size()) - 1 : -1;
+  // hipBLASLt algorithm chosen for this shape (nullptr: rocBLAS)
+  o->sel = h->fnl_algo >= 0 ? h->fnl_algo : nl_builtin_algo(c);
+  LtPlan* fplan = o->sel > 0 ? nl_plan(h, 0, nst, ld, ldh, fp32) : nullptr;
+  const int falgo = fplan ? std::min<int>(o->sel, (int)fplan->cand.size()) - 1 : -1;
   // K slices of kc elements; the slice count actually written (ceil(K / kc)) can be below
   // the requested split count, and the finish kernel sums exactly that many partials.
   // hipBLASLt contracts the full K in one product.
-  const int want =
-      fplan ? 1 : (h->nsplit > 0 ? h->nsplit : fnl_splits(P, nst, Kelem));
+  const int want = fplan ? 1 : (h->nsplit > 0 ? h->nsplit : fnl_splits(P, nst, Kelem));
   const long kc = (Kelem + want - 1) / want;
   const int nsplit = (int)((Kelem + kc - 1) / kc);
   const long pstride = (long)P * nst * e;
@@ -523,7 +523,10 @@ int nl_apply(Ham* h, const double* psi, long ld, int nst, const double* occ_host
   if (!fp32) {
     double* F = static_cast<double*>(grow(h->f64, h->f64_bytes, fsz * sizeof(double)));
     const double* anl = static_cast<const double*>(h->anl64);
-    if (!F) return QBH_ERR_ALLOC;
+    if (!F) {
+      c->last_error = "projector coefficient allocation failed";
+      return QBH_ERR_ALLOC;
+    }
     {
       TimedLaunch tl(c, "fnl_gemm");
       if (int r = fplan ? lt_gemm<double>(c, fplan, falgo, anl, psi, F, 0.0)
@@ -531,34 +534,20 @@ int nl_apply(Ham* h, const double* psi, long ld, int nst, const double* occ_host
                                            real ? 2 * ld : ld, F, kc))
         return r;
     }
-    {
-      TimedLaunch tl(c, "fnl_finish");
-      if (real)
-        k_fnl_finish_real<double><<<nblk, 256, 0, c->stream>>>(P, nst, F, nsplit, pstride, anl,
-                                                               2L * ngw, psi, 2L * ld, h->col_fac,
-                                                               occ, h->red);
-      else
-        k_fnl_finish_cplx<double><<<nblk, 256, 0, c->stream>>>(P, nst, F, nsplit, pstride,
-                                                               h->col_fac, occ, h->red);
-    }
-    TimedLaunch tl(c, "enl_gemm");
-    const double beta = overwrite ? 0.0 : 1.0;
-    if (eplan) {
-      if (int r = lt_gemm<double>(c, eplan, ealgo, anl, F, hpsi, beta)) return r;
-    } else if (int r = blas_ok(c,
-                               gemm_sb(bh, rocblas_operation_none, rocblas_operation_none,
-                                       (int)Kelem, nst, P, anl, Kelem, 0, F, P, 0, hpsi,
-                                       real ? 2 * ldh : ldh, 0, 1, cplx, beta),
-                               "enl gemm")) {
-      return r;
-    }
+    TimedLaunch tl(c, "fnl_finish");
+    if (real)
+      k_fnl_finish_real<double><<<kRedBlocks, 256, 0, c->stream>>>(
+          P, nst, F, nsplit, pstride, anl, 2L * ngw, psi, 2L * ld, h->col_fac, occ, h->red);
+    else
+      k_fnl_finish_cplx<double><<<kRedBlocks, 256, 0, c->stream>>>(P, nst, F, nsplit, pstride,
+                                                                   h->col_fac, occ, h->red);
+    o->F = F;
   } else {
     float* F = static_cast<float*>(grow(h->f32, h->f32_bytes, fsz * sizeof(float)));
     const size_t psz = 2 * (size_t)ngw * nst;
     float* p32 = static_cast<float*>(grow(h->psi32, h->psi32_bytes, psz * sizeof(float)));
-    float* t32 = static_cast<float*>(grow(h->tmp32, h->tmp32_bytes, psz * sizeof(float)));
     const float* anl = static_cast<const float*>(h->anl32);
-    if (!F || !p32 || !t32) {
+    if (!F || !p32) {
       c->last_error = "FP32 projector scratch allocation failed";
       return QBH_ERR_ALLOC;
     }
@@ -574,15 +563,64 @@ int nl_apply(Ham* h, const double* psi, long ld, int nst, const double* occ_host
                                           F, kc))
         return r;
     }
-    {
-      TimedLaunch tl(c, "fnl_finish");
-      if (real)
-        k_fnl_finish_real<float><<<nblk, 256, 0, c->stream>>>(P, nst, F, nsplit, pstride, anl,
-                                                              2L * ngw, p32, 2L * ngw, h->col_fac,
-                                                              occ, h->red);
-      else
-        k_fnl_finish_cplx<float><<<nblk, 256, 0, c->stream>>>(P, nst, F, nsplit, pstride,
-                                                              h->col_fac, occ, h->red);
+    TimedLaunch tl(c, "fnl_finish");
+    if (real)
+      k_fnl_finish_real<float><<<kRedBlocks, 256, 0, c->stream>>>(
+          P, nst, F, nsplit, pstride, anl, 2L * ngw, p32, 2L * ngw, h->col_fac, occ, h->red);
+    else
+      k_fnl_finish_cplx<float><<<kRedBlocks, 256, 0, c->stream>>>(P, nst, F, nsplit, pstride,
+                                                                  h->col_fac, occ, h->red);
+    o->F = F;
+  }
+  return 0;
+}
+
+int nl_apply(Ham* h, const double* psi, long ld, int nst, const double* occ_host, double* hpsi,
+             long ldh, int fp32, double* enl, const float** defer_add32, bool overwrite) {
+  Ctx* c = h->ctx;
+  if (h->P == 0) {
+    if (enl) *enl = 0.0;
+    return 0;
+  }
+  const int ngw = h->b->g.ngw;
+  const bool real = h->b->g.real;
+  const bool cplx = !real;
+  const int P = h->P;
+  if (int r = ensure_anl(h, fp32)) return r;
+  void* hb;
+  if (int r = blas_handle(c, &hb)) return r;
+  rocblas_handle bh = static_cast<rocblas_handle>(hb);
+  const double* occ;
+  if (int r = nl_occ(h, nst, occ_host, &occ)) return r;
+  FnlStage fs;
+  if (int r = fnl_stage(h, bh, psi, ld, nst, ldh, fp32 != 0, occ, &fs)) return r;
+
+  const long Kelem = real ? 2L * ngw : (long)ngw;
+  const int esel = h->enl_algo >= 0 ? h->enl_algo : nl_builtin_algo(c);
+  LtPlan* eplan = esel > 0 ? nl_plan(h, 1, nst, ld, ldh, fp32 != 0) : nullptr;
+  const int ealgo = eplan ? std::min<int>(esel, (int)eplan->cand.size()) - 1 : -1;
+  if (!fp32) {
+    const double* F = static_cast<const double*>(fs.F);
+    const double* anl = static_cast<const double*>(h->anl64);
+    TimedLaunch tl(c, "enl_gemm");
+    const double beta = overwrite ? 0.0 : 1.0;
+    if (eplan) {
+      if (int r = lt_gemm<double>(c, eplan, ealgo, anl, F, hpsi, beta)) return r;
+    } else if (int r = blas_ok(c,
+                               gemm_sb(bh, rocblas_operation_none, rocblas_operation_none,
+                                       (int)Kelem, nst, P, anl, Kelem, 0, F, P, 0, hpsi,
+                                       real ? 2 * ldh : ldh, 0, 1, cplx, beta),
+                               "enl gemm")) {
+      return r;
+    }
+  } else {
+    const float* F = static_cast<const float*>(fs.F);
+    const size_t psz = 2 * (size_t)ngw * nst;
+    float* t32 = static_cast<float*>(grow(h->tmp32, h->tmp32_bytes, psz * sizeof(float)));
+    const float* anl = static_cast<const float*>(h->anl32);
+    if (!t32) {
+      c->last_error = "FP32 projector scratch allocation failed";
+      return QBH_ERR_ALLOC;
     }
     {
       TimedLaunch tl(c, "enl_gemm");
@@ -611,9 +649,198 @@ int nl_apply(Ham* h, const double* psi, long ld, int nst, const double* occ_host
   }
   if (enl) {
     // Enl is reduced on the device; nl_read_enl fetches it once the stream has drained
-    k_sum_partials<<<1, 256, 0, c->stream>>>(nblk, h->red);
+    k_sum_partials<<<1, 256, 0, c->stream>>>(kRedBlocks, h->red);
     *enl = 0.0;
   }
+  return 0;
+}
+
+// ------------------------------------------------------------------------------------
+// Non-local ionic forces (NonLocalPotential.cc:2175-2323).  The reference tables, per
+// direction j, anl_j = (k+G-A)_j twnl (-i)^(l+1) exp(-i (k+G-A).tau) = -i (k+G-A)_j anl
+// and contracts dfnl_j = anl_j^T c (real view, x2 at Γ) or anl_j^H c.  The same numbers
+// come from the resident anl table applied to psi'_j = i (k+G-A)_j psi, since
+// Re(-i k a)^T Re(c) + Im(-i k a)^T Im(c) = Re(a)^T Re(i k c) + Im(a)^T Im(i k c) and
+// (-i k a)^H c = a^H (i k c): one band-sized scaling pass per direction instead of three
+// more P x ngw tables.
+
+// psi'_j = i (k+G-A)_j psi, compact (2 ngw reals per band); grid (ceil(ngw/256), nst)
+template <typename T>
+__global__ void __launch_bounds__(256)
+k_dpsi(int ngw, const double* __restrict__ psi, long ld, const double* __restrict__ kpgxj,
+       double aj, T* __restrict__ out) {
+  const int ig = blockIdx.x * 256 + threadIdx.x;
+  if (ig >= ngw) return;
+  const long n = blockIdx.y;
+  const double2 v = *reinterpret_cast<const double2*>(psi + 2 * (n * ld + ig));
+  const double k = kpgxj[ig] - aj;
+  T* o = out + 2 * (n * ngw + ig);
+  o[0] = T(-k * v.y);
+  o[1] = T(k * v.x);
+}
+
+// One direction: part[s*P + p] = 2 scale sum_{n in chunk s} occ_n F[p,n] dF[p,n] (complex:
+// Re(F conj(dF))), dF first summed over its nsplit K-slice partials.  F holds the wt/Omega
+// scaled coefficients, 2 occ_n is facn of :2281-2282, scale is the x2 of dfnl at Γ
+// (:2265-2267; there is no G=0 term to remove: the derivative factor vanishes at G=0
+// without a vector potential).  Block: 64 columns x 4 band lanes; grid (ceil(P/64), S).
+template <typename T, bool CPLX>
+__global__ void __launch_bounds__(256)
+k_nl_force(int P, int nst, const T* __restrict__ F, const T* __restrict__ dF, int nsplit,
+           long pstride, const double* __restrict__ occ, double scale,
+           double* __restrict__ part) {
+  __shared__ double red[256];
+  const int p = blockIdx.x * 64 + (threadIdx.x & 63);
+  const int lane = threadIdx.x >> 6;
+  double acc = 0.0;
+  if (p < P) {
+    for (int n = blockIdx.y * 4 + lane; n < nst; n += 4 * gridDim.y) {
+      const long i = (long)n * P + p;
+      if (!CPLX) {
+        T d = dF[i];
+        for (int k = 1; k < nsplit; ++k) d += dF[k * pstride + i];
+        acc += occ[n] * (double(F[i]) * double(d));
+      } else {
+        T dr = dF[2 * i], di = dF[2 * i + 1];
+        for (int k = 1; k < nsplit; ++k) {
+          dr += dF[k * pstride + 2 * i];
+          di += dF[k * pstride + 2 * i + 1];
+        }
+        acc += occ[n] * (double(F[2 * i]) * double(dr) + double(F[2 * i + 1]) * double(di));
+      }
+    }
+  }
+  red[threadIdx.x] = acc;
+  __syncthreads();
+  const int t = threadIdx.x;
+  if (t < 64 && p < P)
+    part[(long)blockIdx.y * P + p] = 2.0 * scale * ((red[t] + red[t + 64]) + (red[t + 128] + red[t + 192]));
+}
+
+// g[p] = sum_s part[s*P + p]
+__global__ void __launch_bounds__(256)
+k_col_sum(int P, int S, const double* __restrict__ part, double* __restrict__ g) {
+  const int p = blockIdx.x * 256 + threadIdx.x;
+  if (p >= P) return;
+  double a = 0.0;
+  for (int s = 0; s < S; ++s) a += part[(long)s * P + p];
+  g[p] = a;
+}
+
+template <typename T>
+static int dfnl_forces(Ham* h, rocblas_handle bh, const double* psi, long ld, int nst,
+                       bool fp32, int sel, const T* anl, const T* F, const double* occ, T* dpsi,
+                       double* g) {
+  Ctx* c = h->ctx;
+  const int ngw = h->b->g.ngw;
+  const bool real = h->b->g.real;
+  const bool cplx = !real;
+  const int P = h->P;
+  const long Kelem = real ? 2L * ngw : (long)ngw;
+  const int e = cplx ? 2 : 1;
+  // psi' is compact (leading dimension ngw); same library choice as the fnl product
+  LtPlan* dplan = sel > 0 ? nl_plan(h, 0, nst, ngw, ngw, fp32) : nullptr;
+  const int dalgo = dplan ? std::min<int>(sel, (int)dplan->cand.size()) - 1 : -1;
+  const int want = dplan ? 1 : (h->nsplit > 0 ? h->nsplit : fnl_splits(P, nst, Kelem));
+  const long kc = (Kelem + want - 1) / want;
+  const int nsplit = (int)((Kelem + kc - 1) / kc);
+  const long pstride = (long)P * nst * e;
+  T* dF = static_cast<T*>(grow(h->df, h->df_bytes, (size_t)pstride * nsplit * sizeof(T)));
+  const int S = std::max(1, std::min(64, (nst + 15) / 16));
+  double* part = static_cast<double*>(
+      grow(h->fpart, h->fpart_bytes, (size_t)S * P * sizeof(double)));
+  if (!dF || !part) {
+    c->last_error = "force scratch allocation failed";
+    return QBH_ERR_ALLOC;
+  }
+  for (int j = 0; j < 3; ++j) {
+    {
+      TimedLaunch tl(c, "dpsi");
+      k_dpsi<T><<<dim3((unsigned)((ngw + 255) / 256), (unsigned)nst), 256, 0, c->stream>>>(
+          ngw, psi, ld, h->b->kpgx + (long)j * ngw, h->A[j], dpsi);
+    }
+    {
+      TimedLaunch tl(c, "dfnl_gemm");
+      if (int r = dplan ? lt_gemm<T>(c, dplan, dalgo, anl, dpsi, dF, T(0))
+                        : fnl_gemm<T>(c, bh, cplx, P, nst, Kelem, anl, Kelem, dpsi, Kelem, dF,
+                                      kc))
+        return r;
+    }
+    TimedLaunch tl(c, "nl_force");
+    const dim3 grid((unsigned)((P + 63) / 64), (unsigned)S);
+    if (real)
+      k_nl_force<T, false><<<grid, 256, 0, c->stream>>>(P, nst, F, dF, nsplit, pstride, occ, 2.0,
+                                                        part);
+    else
+      k_nl_force<T, true><<<grid, 256, 0, c->stream>>>(P, nst, F, dF, nsplit, pstride, occ, 1.0,
+                                                       part);
+    k_col_sum<<<(P + 255) / 256, 256, 0, c->stream>>>(P, S, part, g + (long)j * P);
+  }
+  return 0;
+}
+
+int nl_forces(Ham* h, const double* psi, long ld, int nst, const double* occ_host, int fp32,
+              double* fion, double* enl) {
+  Ctx* c = h->ctx;
+  long natoms = 0;
+  for (int s = 0; s < h->nsp; ++s) natoms += h->na[s];
+  std::fill(fion, fion + 3 * natoms, 0.0);
+  if (enl) *enl = 0.0;
+  if (h->P == 0) return 0;
+  const int ngw = h->b->g.ngw;
+  const int P = h->P;
+  if (int r = ensure_anl(h, fp32)) return r;
+  void* hb;
+  if (int r = blas_handle(c, &hb)) return r;
+  rocblas_handle bh = static_cast<rocblas_handle>(hb);
+  const double* occ;
+  if (int r = nl_occ(h, nst, occ_host, &occ)) return r;
+  FnlStage fs;
+  if (int r = fnl_stage(h, bh, psi, ld, nst, ngw, fp32 != 0, occ, &fs)) return r;
+  double* g = static_cast<double*>(grow(h->fg, h->fg_bytes, 3 * (size_t)P * sizeof(double)));
+  if (!g) {
+    c->last_error = "force scratch allocation failed";
+    return QBH_ERR_ALLOC;
+  }
+  const size_t psz = 2 * (size_t)ngw * nst;
+  int r;
+  if (!fp32) {
+    double* dpsi = static_cast<double*>(grow(h->dpsi, h->dpsi_bytes, psz * sizeof(double)));
+    if (!dpsi) {
+      c->last_error = "force scratch allocation failed";
+      return QBH_ERR_ALLOC;
+    }
+    r = dfnl_forces<double>(h, bh, psi, ld, nst, false, fs.sel,
+                            static_cast<const double*>(h->anl64),
+                            static_cast<const double*>(fs.F), occ, dpsi, g);
+  } else {
+    // the FP32 copy of psi is no longer read once F is finished; psi' reuses the product buffer
+    float* dpsi = static_cast<float*>(grow(h->tmp32, h->tmp32_bytes, psz * sizeof(float)));
+    if (!dpsi) {
+      c->last_error = "force scratch allocation failed";
+      return QBH_ERR_ALLOC;
+    }
+    r = dfnl_forces<float>(h, bh, psi, ld, nst, true, fs.sel, static_cast<const float*>(h->anl32),
+                           static_cast<const float*>(fs.F), occ, dpsi, g);
+  }
+  if (r) return r;
+  if (enl) k_sum_partials<<<1, 256, 0, c->stream>>>(kRedBlocks, h->red);
+  hipError_t err = hipGetLastError();
+  if (err != hipSuccess) {
+    c->last_error = std::string("nonlocal forces: ") + hipGetErrorString(err);
+    return QBH_ERR_DEVICE;
+  }
+  std::vector<double> gh(3 * (size_t)P);
+  hipMemcpyAsync(gh.data(), g, gh.size() * sizeof(double), hipMemcpyDeviceToHost, c->stream);
+  if (enl) hipMemcpyAsync(enl, h->red + kRedBlocks, sizeof(double), hipMemcpyDeviceToHost, c->stream);
+  err = hipStreamSynchronize(c->stream);
+  if (err != hipSuccess) {
+    c->last_error = std::string("nonlocal forces: ") + hipGetErrorString(err);
+    return QBH_ERR_DEVICE;
+  }
+  // tmpfion[3*ia+j] -= ... summed over the columns of each atom (:2283-2288)
+  for (int j = 0; j < 3; ++j)
+    for (int p = 0; p < P; ++p) fion[3L * h->col_atom_host[p] + j] -= gh[(size_t)j * P + p];
   return 0;
 }
 
@@ -639,7 +866,7 @@ namespace qbh {
 double nl_read_enl(Ham* h, hipStream_t s) {
   double v = 0.0;
   if (!h->red || h->P == 0) return 0.0;
-  hipMemcpyAsync(&v, h->red + 512, sizeof(double), hipMemcpyDeviceToHost, s);
+  hipMemcpyAsync(&v, h->red + kRedBlocks, sizeof(double), hipMemcpyDeviceToHost, s);
   hipStreamSynchronize(s);
   return v;
 }

@@ -70,6 +70,9 @@ int main(int argc, char** argv) {
   ft.backward(&sd.c()[0], &sd.c()[sd.mloc()], f.data());
   // EnergyFunctional::energy(compute_hpsi=true)
   const double enl = ef.energy(sd, true, dwf);
+  // NonLocalPotential::energy(..., compute_forces=true, fion)
+  std::vector<std::vector<double>> fion;
+  ef.nl_forces(sd, fion);
   // ChargeDensity::update_density
   ChargeDensity cd_(ft, vft, vbasis.localsize());
   cd_.update_density(sd);
@@ -82,6 +85,7 @@ int main(int argc, char** argv) {
   dump(out, "hpsi.bin", dwf.c());
   dump(out, "rho.bin", cd_.rhor);
   dump(out, "rhog.bin", cd_.rhog);
+  dump(out, "fion.bin", fion[0]);
   std::vector<double> sc = {enl, cd_.nelectrons};
   dump(out, "scalars.bin", sc);
   std::printf("adaptor_check: enl %.15f nelectrons %.15f\n", enl, cd_.nelectrons);

@@ -10,7 +10,7 @@
 // Reference interfaces mirrored (qball src/qball/):
 //   FourierTransform::backward/forward           FourierTransform.h:144-153
 //   SlaterDet::rs_mul_add / compute_density / gram  SlaterDet.h:113-126
-//   NonLocalPotential::energy                     NonLocalPotential.h:93-96
+//   NonLocalPotential::energy (Enl, Hψ term, fion)  NonLocalPotential.h:93-96
 //   EnergyFunctional::energy(psi, compute_hpsi, dwf, ...), v_r   EnergyFunctional.h:104-113
 //   ChargeDensity::update_density                 ChargeDensity.h:92
 //   ExponentialWavefunctionStepper::preupdate/update  ExponentialWavefunctionStepper.h
@@ -298,6 +298,26 @@ class EnergyFunctional {
                                  wt.data(), twnl.data(), tau.data(),
                                  ft_.basis().cell().volume),
           ft_.context().get(), "qbh_ham_set_projectors");
+    na_ = na;
+  }
+  // non-local ionic forces, NonLocalPotential::energy(sd, ..., compute_forces=true, fion)
+  // (NonLocalPotential.cc:2175-2323): fion[is][3*ia+j] overwritten; returns Enl
+  double nl_forces(const SlaterDet& psi, std::vector<std::vector<double>>& fion,
+                   int flags = 0) {
+    size_t natoms = 0;
+    for (int n : na_) natoms += (size_t)n;
+    std::vector<double> f(3 * natoms);
+    double enl = 0.0;
+    check(qbh_nl_forces(h_, psi.dev_c(), psi.mloc(), psi.nst(), psi.occ().data(), f.data(),
+                        flags, &enl),
+          ft_.context().get(), "qbh_nl_forces");
+    fion.resize(na_.size());
+    size_t a0 = 0;
+    for (size_t is = 0; is < na_.size(); ++is) {
+      fion[is].assign(f.begin() + 3 * a0, f.begin() + 3 * (a0 + na_[is]));
+      a0 += (size_t)na_[is];
+    }
+    return enl;
   }
   void set_vector_potential(const double A[3]) {
     check(qbh_ham_set_vector_potential(h_, A), ft_.context().get(), "set_vector_potential");
@@ -330,6 +350,7 @@ class EnergyFunctional {
   FourierTransform& ft_;
   qbh_ham h_ = nullptr;
   DevBuf dv_;
+  std::vector<int> na_;
 };
 
 inline void SlaterDet::rs_mul_add(FourierTransform& ft, const double* v, SlaterDet& sdp) const {

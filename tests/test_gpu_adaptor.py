@@ -1,6 +1,6 @@
 """The qball-shaped C++ adaptor (qball_amd/host/qbh_qball.h) end to end on the GPU:
-FourierTransform::backward, EnergyFunctional::energy(compute_hpsi=true) and
-ChargeDensity::update_density against the CPU oracle on the driver's own inputs."""
+FourierTransform::backward, EnergyFunctional::energy(compute_hpsi=true), the non-local
+forces and ChargeDensity::update_density against the CPU oracle on the driver's own inputs."""
 import math
 import os
 import subprocess
@@ -45,6 +45,9 @@ def test_adaptor_matches_oracle(tmp_path):
     assert rel_err(h, ref) <= 1e-12
     sc = rd("scalars.bin", np.float64)
     assert abs(sc[0] - enl) <= 1e-10
+    # non-local ionic forces
+    _, fref = oh.nonlocal_forces(b, [sp], psi, occ, cell.volume)
+    assert rel_err(rd("fion.bin", np.float64).reshape(8, 3), fref[0]) <= 1e-12
     # update_density: rho, nelectrons, rhog
     rho, nel, rhog = oh.update_density(maps, vmaps, psi, occ, 1.0, cell.volume)
     assert rel_err(rd("rho.bin", np.float64), rho) <= 1e-12

@@ -69,3 +69,59 @@ def test_full_size_hpsi(gpu_ctx, which):
     assert np.linalg.norm(m - m.T) <= 1e-12 * np.linalg.norm(m)
     ham.close()
     basis.close()
+
+
+@pytest.mark.parametrize("which", ["si512", "al500"])
+def test_full_size_nl_forces(gpu_ctx, which):
+    """Non-local forces at the benchmark sizes: additivity over bands, the FP32 form against
+    FP64, -dEnl/dtau of the library's own Enl by central differences for one atom, and (Si512)
+    the oracle on a band subset."""
+    from oracle import hpsi as oh
+    from qball_amd import Basis, Hamiltonian
+    s = _system(which)
+    r = s.rods
+    basis = Basis(gpu_ctx, True, r["rod_h"], r["rod_k"], r["rod_lmin"], r["rod_size"], s.np,
+                  s.b)
+    ham = Hamiltonian(gpu_ctx, basis)
+
+    def place(tau):
+        ham.set_projectors([s.na], [len(s.lproj)], s.lproj, s.wt, s.twnl.ravel(), tau.ravel(),
+                           s.omega)
+
+    place(s.tau)
+    ngw, nst = s.ngw, s.nst
+    cd = to_dev(s.psi)
+    f, enl = ham.nl_forces(cd.data_ptr(), ngw, nst, s.occ, want_enl=True)
+    f32 = ham.nl_forces(cd.data_ptr(), ngw, nst, s.occ, fp32=True)
+    fmax = np.max(np.abs(f))
+    print(which, "Enl", enl, "max|F|", fmax, "fp32 vs fp64", rel_err(f32, f), flush=True)
+    assert rel_err(f32, f) <= 1e-5
+    # forces are sums over bands: two halves add up to the whole
+    h = nst // 2
+    fa = ham.nl_forces(cd.data_ptr(), ngw, h, s.occ[:h])
+    fb = ham.nl_forces(cd.data_ptr() + 16 * ngw * h, ngw, nst - h, s.occ[h:])
+    assert rel_err(fa + fb, f) <= 1e-12
+    # F = -dEnl/dtau, central differences of the library's Enl (step 2e-4 bohr)
+    step = 2e-4
+    ia = 1
+    for j in range(3):
+        e = []
+        for sg in (1, -1):
+            tau = s.tau.copy()
+            tau[ia, j] += sg * step
+            place(tau)
+            e.append(ham.nl_forces(cd.data_ptr(), ngw, nst, s.occ, want_enl=True)[1])
+        fd = -(e[0] - e[1]) / (2 * step)
+        print(which, "atom", ia, "dir", j, "F", f[ia, j], "fd", fd, flush=True)
+        assert abs(fd - f[ia, j]) <= 1e-5 * fmax
+    place(s.tau)
+    if which == "si512":
+        sub = [0, 1, nst - 2, nst - 1]
+        ob, _, sp = oracle_for(s)
+        enl_ref, fref = oh.nonlocal_forces(ob, [sp], s.psi[:, sub], s.occ[sub], s.omega)
+        c4 = to_dev(s.psi[:, sub])
+        f4, enl4 = ham.nl_forces(c4.data_ptr(), ngw, 4, s.occ[sub], want_enl=True)
+        assert rel_err(f4, fref[0]) <= 1e-12
+        assert abs(enl4 - enl_ref) <= 1e-10 * max(1.0, abs(enl_ref))
+    ham.close()
+    basis.close()

@@ -114,6 +114,80 @@ def test_projector_blocks_do_not_change_result():
     assert abs(e1 - e2) <= 1e-13 * abs(e1) and np.max(np.abs(cp1 - cp2)) <= 1e-13
 
 
+def species_all_l(b, rng, na=3):
+    """One projector per l = 0..3 with smooth radial tables (the force identities below
+    need no particular table, only one per l pattern of the phase)."""
+    g = b.kpg
+    twnl = np.stack([(1.0 + 0.3 * l) * g ** l * np.exp(-0.35 * g * g) for l in range(4)])
+    tau = rng.uniform(0, 5, size=(na, 3))
+    return oh.Projectors([0, 1, 2, 3], [1.3, -0.7, 0.4, 0.25], twnl, tau)
+
+
+def _fd_forces(b, sp, c, occ, omega, kpgx, h=1e-5):
+    """-dEnl/dtau by central differences of the oracle's Enl."""
+    f = np.zeros((sp.na, 3))
+    for ia in range(sp.na):
+        for j in range(3):
+            e = []
+            for s in (1, -1):
+                tau = sp.tau.copy()
+                tau[ia, j] += s * h
+                p = oh.Projectors(sp.lproj, sp.wt, sp.twnl, tau)
+                e.append(oh.nonlocal_energy(b, [p], c, occ, omega, kpgx=kpgx))
+            f[ia, j] = -(e[0] - e[1]) / (2 * h)
+    return f
+
+
+@pytest.mark.parametrize("real,vp", [(True, False), (False, False), (False, True)])
+def test_nonlocal_forces_are_minus_enl_gradient(real, vp):
+    """NonLocalPotential.cc:2175-2323: fion = -dEnl/dtau for every l pattern, Γ and complex,
+    and with a vector potential (phases and derivative factors with k+G-A)."""
+    cell, b, maps, c, rng = setup(real, 4, seed=3)
+    sp = species_all_l(b, rng)
+    occ = np.array([2.0, 2.0, 1.0, 0.5])
+    kpgx = b.kpgx - np.array([0.11, -0.07, 0.05]) if vp else b.kpgx
+    enl, fion = oh.nonlocal_forces(b, [sp], c, occ, cell.volume, kpgx=kpgx)
+    assert enl == oh.nonlocal_energy(b, [sp], c, occ, cell.volume, kpgx=kpgx)
+    fd = _fd_forces(b, sp, c, occ, cell.volume, kpgx)
+    assert np.max(np.abs(fion[0] - fd)) <= 1e-7 * np.max(np.abs(fd))
+
+
+def test_nonlocal_forces_blocks_and_species():
+    """Atom blocks (na_block) only reorder sums; two species give the forces of each alone."""
+    cell, b, maps, c, rng = setup(True, 3, seed=5)
+    s1 = species_all_l(b, rng, na=5)
+    s2 = species(b, rng, na=2)
+    occ = np.array([2.0, 1.5, 0.0])
+    e, f = oh.nonlocal_forces(b, [s1, s2], c, occ, cell.volume)
+    e1, f1 = oh.nonlocal_forces(b, [s1], c, occ, cell.volume, na_block=2)
+    e2, f2 = oh.nonlocal_forces(b, [s2], c, occ, cell.volume)
+    assert abs(e - (e1 + e2)) <= 1e-13 * abs(e)
+    assert np.max(np.abs(f[0] - f1[0])) <= 1e-13 * np.max(np.abs(f1[0]))
+    assert np.max(np.abs(f[1] - f2[0])) <= 1e-13 * np.max(np.abs(f2[0]))
+
+
+@pytest.mark.parametrize("real,l", [(True, 2), (False, 3)])
+def test_nonlocal_force_check_catches_a_wrong_phase(monkeypatch, real, l):
+    """A derivative table whose l pattern has its phase off by i (a swapped real/imaginary
+    part) fails the gradient check: the identity pins each l pattern, not only their sum."""
+    cell, b, maps, c, rng = setup(real, 3, seed=7)
+    sp = species_all_l(b, rng, na=2)
+    occ = np.array([2.0, 1.0, 1.0])
+    fd = _fd_forces(b, sp, c, occ, cell.volume, b.kpgx)
+    good = oh.danl_block
+
+    def planted(sp_, kpgx, j, ia0, ia1):
+        d = good(sp_, kpgx, j, ia0, ia1)
+        nblk = ia1 - ia0
+        cols = slice(l * nblk, (l + 1) * nblk)     # projector ipr = l has angular momentum l
+        d[:, cols] = 1j * d[:, cols]
+        return d
+
+    monkeypatch.setattr(oh, "danl_block", planted)
+    _, f = oh.nonlocal_forces(b, [sp], c, occ, cell.volume)
+    assert np.max(np.abs(f[0] - fd)) > 1e-3 * np.max(np.abs(fd))
+
+
 @pytest.mark.parametrize("real", [True, False])
 def test_density_integral_and_direct_sum(real):
     cell, b, maps, c, rng = setup(real, 4)
