@@ -246,6 +246,13 @@ def main():
            "equivalent_frac_of_peak": b_model / t_fft / 1e6 / PEAK_HBM_GBS,
            "note": "five transform passes of one FP64 Hψ, sequential characterisation pass; "
                    "dram_bytes from the committed FETCH/WRITE counter runs"}
+    # what plain streaming kernels reach on this GPU (2 GiB buffers): the practical ceiling
+    # for the HBM rate of a pass that reads and writes
+    sb = ctx.stream_bandwidth(2.0, 3)
+    fft["stream_kernels_GBps"] = sb
+    fft["frac_of_stream_copy"] = fft["GBps"] / sb["copy"]
+    fft["dram_frac_of_stream_copy"] = (fft["dram_GBps"] / sb["copy"]) if b_dram else None
+    log(f"stream kernels: read {sb['read']:.0f} write {sb['write']:.0f} copy {sb['copy']:.0f} GB/s")
     roof["measured_in"] = ("sequential characterisation pass at the built-in launch choices, "
                            f"{char_steps} steps, HIP events on the library stream")
 

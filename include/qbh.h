@@ -65,6 +65,12 @@ int qbh_ctx_timing_report(qbh_ctx ctx, char* buf, int cap);
  * library the built-in projector GEMM choice uses with them (no counterpart in the
  * reference; diagnostics). */
 int qbh_ctx_library_info(qbh_ctx ctx, char* buf, int cap);
+/* Stream bandwidth of the context's device (no counterpart in the reference; diagnostics):
+ * grid-stride read-only, write-only and copy kernels over two temporary buffers of `bytes`
+ * (1 MiB .. 64 GiB), best of `reps` timed launches each; gbps[0..2] = read, write and copy
+ * rates in GB/s, the copy rate counting read plus written bytes.  The practical ceiling
+ * for the HBM rate of a streaming pass.  Synchronises the stream. */
+int qbh_ctx_stream_bandwidth(qbh_ctx ctx, double bytes, int reps, double* gbps);
 
 /* ---- plane-wave basis ------------------------------------------------------------------
  * Replaces the index maps that FourierTransform(basis, np0, np1, np2) builds from a qball

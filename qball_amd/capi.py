@@ -33,6 +33,7 @@ SIGNATURES = {
     "qbh_ctx_set_overlap": (_c_int, [_vp, _c_int]),
     "qbh_ctx_timing_report": (_c_int, [_vp, ctypes.c_char_p, _c_int]),
     "qbh_ctx_library_info": (_c_int, [_vp, ctypes.c_char_p, _c_int]),
+    "qbh_ctx_stream_bandwidth": (_c_int, [_vp, _c_double, _c_int, _dp]),
     "qbh_basis_create": (_c_int, [_vp, _c_int, _c_int, _ip, _ip, _ip, _ip, _c_int, _c_int,
                                   _c_int, _dp, _dp, ctypes.POINTER(_vp)]),
     "qbh_basis_destroy": (_c_int, [_vp]),
@@ -192,6 +193,14 @@ class Context:
         buf = ctypes.create_string_buffer(512)
         self.check(lib().qbh_ctx_library_info(self.h, buf, len(buf)), "library_info")
         return buf.value.decode()
+
+    def stream_bandwidth(self, gib=2.0, reps=3):
+        """GB/s of grid-stride read-only, write-only and copy (read + written bytes) kernels
+        over `gib` GiB buffers on this context's device."""
+        out = (_c_double * 3)()
+        self.check(lib().qbh_ctx_stream_bandwidth(self.h, float(gib) * (1 << 30), int(reps), out),
+                   "stream_bandwidth")
+        return {"read": out[0], "write": out[1], "copy": out[2]}
 
     def close(self):
         if self.h:

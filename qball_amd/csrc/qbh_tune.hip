@@ -400,3 +400,81 @@ extern "C" int qbh_batch_launch(qbh_batch bt) {
   if (e != hipSuccess) return ufail(bt->user, QBH_ERR_DEVICE, std::string("graph launch: ") + hipGetErrorString(e));
   return QBH_OK;
 }
+
+// ------------------------------------------------------------------------------------
+// Stream bandwidth of the device: grid-stride read-only, write-only and copy kernels over
+// two buffers of `bytes`, 32-byte elements, best of `reps` launches each.  The copy rate
+// counts read plus written bytes.  What a plain streaming pass reaches on this GPU is the
+// practical ceiling against which the transform passes' HBM rates are read.
+namespace qbh {
+__global__ void __launch_bounds__(256)
+k_bw_read(const double4* __restrict__ a, long n, double* __restrict__ sink) {
+  double s = 0.0;
+  for (long i = blockIdx.x * 256L + threadIdx.x; i < n; i += (long)gridDim.x * 256) {
+    const double4 v = a[i];
+    s += (v.x + v.y) + (v.z + v.w);
+  }
+  if (s == -1.0) sink[0] = s;  // the buffer holds non-negative values: never stored
+}
+__global__ void __launch_bounds__(256) k_bw_write(double4* __restrict__ a, long n) {
+  for (long i = blockIdx.x * 256L + threadIdx.x; i < n; i += (long)gridDim.x * 256)
+    a[i] = make_double4(1.0, 2.0, 3.0, 4.0);
+}
+__global__ void __launch_bounds__(256)
+k_bw_copy(const double4* __restrict__ a, double4* __restrict__ b, long n) {
+  for (long i = blockIdx.x * 256L + threadIdx.x; i < n; i += (long)gridDim.x * 256) b[i] = a[i];
+}
+}  // namespace qbh
+
+extern "C" int qbh_ctx_stream_bandwidth(qbh_ctx ctx, double bytes, int reps, double* gbps) {
+  using namespace qbh;
+  if (!ctx || !gbps || reps < 1 || !(bytes >= 1048576.0) || bytes > 68719476736.0)
+    return QBH_ERR_ARG;
+  Ctx* c = ctx_of(ctx);
+  hipSetDevice(c->device);
+  const long n = (long)(bytes / sizeof(double4));
+  const size_t nb = (size_t)n * sizeof(double4);
+  void *a = nullptr, *b = nullptr, *sink = nullptr;
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  int r = QBH_OK;
+  if (hipMalloc(&a, nb) != hipSuccess || hipMalloc(&b, nb) != hipSuccess ||
+      hipMalloc(&sink, 64) != hipSuccess) {
+    r = ufail(c, QBH_ERR_ALLOC, "bandwidth buffers allocation failed");
+  } else {
+    hipEventCreate(&e0);
+    hipEventCreate(&e1);
+    // a holds non-negative values (read and copied), b is written
+    k_bw_write<<<2048, 256, 0, c->stream>>>(static_cast<double4*>(a), n);
+    float best[3] = {1e30f, 1e30f, 1e30f};
+    for (int rep = 0; rep <= reps && r == QBH_OK; ++rep) {  // launch 0 of each is a warm-up
+      for (int k = 0; k < 3; ++k) {
+        hipEventRecord(e0, c->stream);
+        if (k == 0) k_bw_read<<<16384, 256, 0, c->stream>>>(static_cast<const double4*>(a), n,
+                                                           static_cast<double*>(sink));
+        if (k == 1) k_bw_write<<<16384, 256, 0, c->stream>>>(static_cast<double4*>(b), n);
+        if (k == 2) k_bw_copy<<<16384, 256, 0, c->stream>>>(static_cast<const double4*>(a),
+                                                           static_cast<double4*>(b), n);
+        hipEventRecord(e1, c->stream);
+        if (hipEventSynchronize(e1) != hipSuccess) {
+          r = ufail(c, QBH_ERR_DEVICE, "bandwidth kernels failed");
+          break;
+        }
+        float t = 0.f;
+        hipEventElapsedTime(&t, e0, e1);
+        if (rep > 0 && t < best[k]) best[k] = t;
+      }
+    }
+    if (r == QBH_OK) {
+      gbps[0] = nb / (best[0] * 1e-3) / 1e9;
+      gbps[1] = nb / (best[1] * 1e-3) / 1e9;
+      gbps[2] = 2.0 * nb / (best[2] * 1e-3) / 1e9;
+    }
+  }
+  if (e0) hipEventDestroy(e0);
+  if (e1) hipEventDestroy(e1);
+  hipStreamSynchronize(c->stream);
+  for (void* p : {a, b, sink})
+    if (p) hipFree(p);
+  (void)hipGetLastError();
+  return r;
+}

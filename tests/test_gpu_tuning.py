@@ -279,3 +279,15 @@ def test_wide_sphere_uses_full_line_count(gpu_ctx, flags, tol):
     assert rel_err(from_dev(hd, basis.ngw, s.nst), ref) <= tol
     ham.close()
     basis.close()
+
+
+def test_stream_bandwidth_is_plausible(gpu_ctx):
+    """The stream-bandwidth diagnostic returns read/write/copy rates between 1 TB/s and the
+    8 TB/s HBM peak, and rejects sizes outside its range."""
+    from qball_amd import QbhError
+    bw = gpu_ctx.stream_bandwidth(0.5, 2)
+    print(bw)
+    for k in ("read", "write", "copy"):
+        assert 1000.0 < bw[k] < 8000.0 * 1.02, (k, bw)
+    with pytest.raises(QbhError):
+        gpu_ctx.stream_bandwidth(1e-4, 1)
