@@ -84,6 +84,8 @@ int nl_lt_candidates(Ham* h, int which, int nst, long ld, long ldh, int fp32);
 // the algorithm value the built-in choice (-1) of fnl_algo / enl_algo stands for
 int nl_builtin_algo(Ctx* c);
 int blas_handle(Ctx* c, void** out);
+// frees a context's rocBLAS handle and hipBLASLt state (qbh_nl.hip)
+void blas_destroy(Ctx* c);
 void* grow(void*& p, size_t& have, size_t need);
 
 }  // namespace qbh

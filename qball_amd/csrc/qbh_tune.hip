@@ -262,7 +262,7 @@ struct qbh_batch_s {
 };
 
 static void destroy_branch(Ctx* b) {
-  if (b->blas) rocblas_destroy_handle(static_cast<rocblas_handle>(b->blas));
+  blas_destroy(b);  // rocBLAS handle and hipBLASLt state
   if (b->ws) hipFree(b->ws);
   if (b->side) hipStreamDestroy(b->side);
   if (b->ev_fork) hipEventDestroy(b->ev_fork);
