@@ -468,8 +468,9 @@ def batch_bench(args):
 def large_cell_bench(args):
     """fcc Al500 Γ (a = 38.25 bohr, ecut 30 Ry: 77666 plane waves, 900 bands, 144^3 grid,
     500 x 9 s+p+d projectors): FP64 and FP32-bulk Hψ over all bands with the built-in
-    launch choices, then the FP64 subspace diagonalisation of the FP32-bulk result (the
-    large-cell form: FP32-bulk Hψ + FP64 subspace)."""
+    launch choices and then with the autotuner's (unless --no-tune), then the FP64 subspace
+    diagonalisation of the FP32-bulk result (the large-cell form: FP32-bulk Hψ + FP64
+    subspace)."""
     from qball_amd import QBH_ALL, QBH_FP32, Basis, Context, Hamiltonian
     from qball_amd.devmem import DeviceArray, synchronize
     from qball_amd.workload import make_fcc_al
@@ -488,7 +489,9 @@ def large_cell_bench(args):
     hpsi = DeviceArray(psi.nbytes)
     out = {"workload": f"fcc Al{s.na} Γ, a=38.25 bohr, ecut 30 Ry, {s.ngw} PWs, {s.nst} bands, "
                        f"grid {s.np[0]}^3, {s.na * len(s.lproj)} KB projectors (s+p+d)"}
-    for name, fl in (("fp64", QBH_ALL), ("fp32_bulk", QBH_ALL | QBH_FP32)):
+    knobs = ("zy_reg", "x_reg", "x_spw", "radix", "nsplit", "overlap", "fnl_algo", "enl_algo")
+
+    def timed():
         for _ in range(2):
             ham.hpsi(psi.ptr, s.ngw, s.nst, s.occ, hpsi.ptr, s.ngw, flags=fl)
         synchronize()
@@ -496,8 +499,22 @@ def large_cell_bench(args):
         for _ in range(args.steps):
             ham.hpsi(psi.ptr, s.ngw, s.nst, s.occ, hpsi.ptr, s.ngw, flags=fl)
         synchronize()
-        dt = (time.perf_counter() - t) / args.steps
-        out[name] = {"ms_per_hpsi": dt * 1e3, "band_applications_per_s": s.nst / dt}
+        return (time.perf_counter() - t) / args.steps
+
+    for name, fl in (("fp64", QBH_ALL), ("fp32_bulk", QBH_ALL | QBH_FP32)):
+        for k in knobs:
+            ham.set_tuning(k, -1 if k in ("overlap", "fnl_algo", "enl_algo") else
+                           (1 if k in ("zy_reg", "x_reg") else 0))
+        dt = dt_builtin = timed()
+        choice = None
+        if not args.no_tune:
+            # launch choices for this shape from the library's autotuner (untimed)
+            ham.autotune(psi.ptr, s.ngw, s.nst, s.occ, hpsi.ptr, s.ngw, flags=fl, reps=2,
+                         force=True)
+            choice = {k: ham.get_tuning(k) for k in knobs}
+            dt = timed()
+        out[name] = {"ms_per_hpsi": dt * 1e3, "band_applications_per_s": s.nst / dt,
+                     "builtin_ms_per_hpsi": dt_builtin * 1e3, "tuned_choice": choice}
     # FP64 subspace diagonalisation of the FP32-bulk Hψ (eigenvalues only; psi unchanged)
     synchronize()
     t = time.perf_counter()
