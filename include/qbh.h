@@ -177,7 +177,8 @@ int qbh_update_vhxc(qbh_basis vbasis, const double* rhor_dev, const double* rhog
 
 /* One exponential of the ETRS propagator (ExponentialWavefunctionStepper::exponential,
  * ExponentialWavefunctionStepper.cc:51-148): psi <- sum_{N<=order} (-i dt)^N/N! H^N psi
- * in place, complex basis.  flags as qbh_hpsi (precision bit honoured). */
+ * in place, complex basis.  flags: the Hψ terms as for qbh_hpsi, all three when none is
+ * named, and QBH_FP32 for the FP32-bulk Hψ. */
 int qbh_exponential(qbh_ham ham, double* psi_dev, long ld, int nst, double dt, int order,
                     int flags);
 
@@ -327,7 +328,8 @@ int qbh_td_create(qbh_ham ham, qbh_basis vbasis, int nst, const double* occ, dou
 int qbh_td_destroy(qbh_td td);
 /* psi0_dev (not modified); A: host 3*(nsteps+1); w: host nsteps+1; e: host 3 (direction);
  * vext_dev: nullable, np0*np1*np2 doubles, borrowed until the matching backward;
- * checkpoint: steps between stored psi; flags: QBH_FP32 selects the FP32-bulk Hψ.
+ * checkpoint: steps between stored psi; flags: QBH_FP32 selects the FP32-bulk Hψ (all
+ * three terms of H either way).
  * J receives the objective; psi_out_dev (nullable) psi_nsteps. */
 int qbh_td_forward(qbh_td td, const double* psi0_dev, long ld, int nsteps, double dt, int order,
                    const double* A, const double* w, const double* e, const double* vext_dev,

@@ -518,7 +518,9 @@ int qbh::expo_impl(Ham* h, const double* in, long ldin, double* outp, long ldout
   double fr = 1.0, fi = 0.0;  // factor
   const double* x = in;
   long ldx = ldin;
-  const int hflags = (flags & (QBH_ALL | QBH_FP32)) == 0 ? QBH_ALL : (flags & (QBH_ALL | QBH_FP32));
+  // the terms of H named in flags (all of them when none is named) and the precision bit
+  const int terms = flags & QBH_ALL;
+  const int hflags = (terms ? terms : QBH_ALL) | (flags & QBH_FP32);
   for (int N = 1; N <= order; ++N) {
     // factor *= -i*dt/N (ExponentialWavefunctionStepper.cc:97-100)
     const double m = dt / double(N);

@@ -39,7 +39,8 @@ def td_setup():
                vext=torch.from_numpy(vext.copy()).cuda())
     td = TDPropagator(ham, vb, len(occ), occ, 1.0, sysm.omega, dev["vion"].data_ptr(),
                       dev["rhops"].data_ptr())
-    yield dict(sysm=sysm, tds=tds, psi=psi, vext=vext, ctx=ctx, basis=basis, td=td, dev=dev)
+    yield dict(sysm=sysm, tds=tds, psi=psi, vext=vext, ctx=ctx, basis=basis, ham=ham, td=td,
+               dev=dev)
     td.close()
     ham.close()
     vb.close()
@@ -141,3 +142,46 @@ def test_torch_autograd_function(td_setup):
     assert abs(float(J) - J2) <= 1e-13 * abs(J2)
     assert rel_err(vext.grad.cpu().numpy(), gv.cpu().numpy()) <= 1e-13
     assert abs(float(A0.grad) - float((gA * (A / 0.05)).sum())) <= 1e-12 * abs(float(A0.grad))
+
+
+def test_fp32_exponential_applies_the_whole_hamiltonian(td_setup):
+    """QBH_FP32 alone selects the FP32-bulk Hψ with all three terms of H (the FP64 form's
+    exponential to the FP32 bar), not an empty Hamiltonian."""
+    import torch
+    from qball_amd import QBH_FP32
+    s = td_setup
+    ham, ngw = s["ham"], s["basis"].ngw
+    nst = s["sysm"].nst
+    out = {}
+    for fl in (0, QBH_FP32):
+        p = s["dev"]["psi"].clone()
+        ham.exponential(p.data_ptr(), ngw, nst, 0.1, 4, flags=fl)
+        s["ctx"].sync()
+        out[fl] = p.cpu().numpy()
+    p0 = s["dev"]["psi"].cpu().numpy()
+    assert rel_err(out[0], p0) > 1e-3                  # H is applied
+    assert rel_err(out[QBH_FP32], out[0]) <= 2e-6
+
+
+def test_td_fp32_forward_tracks_fp64(td_setup):
+    """The FP32-bulk propagation stays within FP32 rounding of the FP64 one over a few
+    steps, objective and gradients."""
+    import torch
+    from qball_amd import QBH_FP32
+    s = td_setup
+    nsteps, dt = 4, 0.1
+    A, w = _schedule(nsteps, dt)
+    e = np.array([1.0, 0.2, 0.0])
+    res = {}
+    for fl in (0, QBH_FP32):
+        J = s["td"].forward(s["dev"]["psi"].data_ptr(), s["basis"].ngw, nsteps, dt, A, w, e,
+                            vext_ptr=s["dev"]["vext"].data_ptr(), checkpoint=2, flags=fl)
+        gv = torch.zeros(s["tds"].maps.N, dtype=torch.float64, device="cuda")
+        gA = s["td"].backward(gv.data_ptr())
+        s["ctx"].sync()
+        res[fl] = (J, gv.cpu().numpy(), gA)
+    (J64, gv64, gA64), (J32, gv32, gA32) = res[0], res[QBH_FP32]
+    print("FP32 vs FP64: J", abs(J32 - J64) / abs(J64), "dJ/dv", rel_err(gv32, gv64), "dJ/dA",
+          rel_err(gA32, gA64))
+    assert abs(J32 - J64) <= 1e-5 * abs(J64)
+    assert rel_err(gv32, gv64) <= 1e-4 and rel_err(gA32, gA64) <= 1e-4
