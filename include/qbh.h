@@ -240,8 +240,8 @@ int qbh_mixer_update(qbh_mixer mixer, const double* x_dev, const double* f_dev, 
  * (Kerker rcut rc_kerker, wls rc1 = 3, coefficient mix_alpha, mix_ndim previous pairs,
  * BOSampleStepper.cc:394-447, 987-1046), update_rhor, update_vhxc (into ham's v_r), then
  * nite x (Hψ -> PSDA update -> gram).  The last inner iteration of each cycle uses
- * flags_last, the others flags_inner (e.g. QBH_ALL|QBH_FP32 then QBH_ALL: one FP64
- * residual-correction pass per cycle).  history (nullable, host) receives
+ * flags_last, the others flags_inner (e.g. QBH_FP32 then 0: one FP64 residual-correction
+ * pass per cycle; the Hψ terms named in the flags, all three when none is named).  history (nullable, host) receives
  * nitscf*max(nite,1) rows of {etotal, ekin, enl, eps, ehart, exc, theta}; etotal excludes
  * the ion-ion terms esr - eself, which do not depend on the orbitals. */
 int qbh_scf_run(qbh_ham ham, qbh_basis vbasis, double* psi_dev, long ld, int nst,
@@ -267,9 +267,10 @@ int qbh_ham_set_tuning(qbh_ham ham, const char* knob, int value);
 int qbh_ham_get_tuning(qbh_ham ham, const char* knob, int* value);
 /* Times candidate knob values for this Hψ shape (reps applications each, HIP events on the
  * context's stream), keeps the fastest, and caches the choice per shape in the context
- * (force != 0 re-tunes).  knob_mask: bit k set = tune knob k in the order listed above
- * (0 = all); the others keep their current values.  hpsi_dev is overwritten.  report
- * (nullable) receives the timings. */
+ * (force != 0 re-tunes).  flags: the Hψ terms (all three when none is named) and
+ * QBH_FP32.  knob_mask: bit k set = tune knob k in the order listed above (0 = all); the
+ * others keep their current values.  hpsi_dev is overwritten.  report (nullable) receives
+ * the timings. */
 int qbh_ham_autotune(qbh_ham ham, const double* psi_dev, long ld, int nst, const double* occ,
                      double* hpsi_dev, long ldh, int flags, int reps, int force, int knob_mask,
                      char* report, int cap);

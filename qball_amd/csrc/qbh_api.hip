@@ -518,9 +518,7 @@ int qbh::expo_impl(Ham* h, const double* in, long ldin, double* outp, long ldout
   double fr = 1.0, fi = 0.0;  // factor
   const double* x = in;
   long ldx = ldin;
-  // the terms of H named in flags (all of them when none is named) and the precision bit
-  const int terms = flags & QBH_ALL;
-  const int hflags = (terms ? terms : QBH_ALL) | (flags & QBH_FP32);
+  const int hflags = whole_h_flags(flags);
   for (int N = 1; N <= order; ++N) {
     // factor *= -i*dt/N (ExponentialWavefunctionStepper.cc:97-100)
     const double m = dt / double(N);
@@ -584,7 +582,7 @@ extern "C" int qbh_etrs_step(qbh_ham ham, qbh_basis vbasis, double* psi_dev, lon
   double* tilde = half + 2 * (size_t)g.ngw * nst;
   double* rho = tilde + 2 * (size_t)g.ngw * nst;
   double* rhog = rho + N;
-  const int fl = flags & (QBH_ALL | QBH_FP32);
+  const int fl = whole_h_flags(flags);
   if (int r = expo_impl(h, psi_dev, ld, half, g.ngw, nst, 0.5 * dt, order, fl, nullptr)) return r;
   if (int r = expo_impl(h, half, g.ngw, tilde, g.ngw, nst, 0.5 * dt, order, fl, nullptr)) return r;
   fill0(c, rho, N);

@@ -2,6 +2,7 @@
 #pragma once
 #include <vector>
 
+#include "../../include/qbh.h"
 #include "qbh_internal.h"
 
 namespace qbh {
@@ -61,6 +62,14 @@ struct Ham {
   void* etrs = nullptr;           // ETRS step: psi(t+dt/2), psi~(t+dt), rho, rhog
   size_t etrs_bytes = 0;
 };
+
+// Hψ flags of the entry points that apply H as a whole (exponential, ETRS step, TD
+// propagator, SCF loop, autotuner, batch): the terms named in flags, all three when none
+// is named, and the precision bit
+inline int whole_h_flags(int flags) {
+  const int terms = flags & QBH_ALL;
+  return (terms ? terms : QBH_ALL) | (flags & QBH_FP32);
+}
 
 // Hψ of qbh_hpsi without argument checks beyond leading dimensions (qbh_api.hip)
 int hpsi_impl(Ham* h, const double* psi, long ld, int nst, const double* occ, double* hpsi,

@@ -581,7 +581,7 @@ extern "C" int qbh_scf_run(qbh_ham ham, qbh_basis vbasis, double* psi_dev, long 
       break;
     qbh_psda_preprocess(psda);
     for (int ite = 0; ite < nin && r == QBH_OK; ++ite) {
-      const int fl = (ite == nin - 1) ? flags_last : flags_inner;
+      const int fl = whole_h_flags((ite == nin - 1) ? flags_last : flags_inner);
       double enl = 0.0, ekin = 0.0, th = 0.0;
       if ((r = hpsi_impl(h, psi_dev, ld, nst, occ, hc, g.ngw, fl, &enl))) break;
       if ((r = qbh_kinetic_energy(wb, psi_dev, ld, nst, occ, &ekin))) break;

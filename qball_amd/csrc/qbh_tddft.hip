@@ -316,7 +316,7 @@ static int step_forward(qbh_td_s* t, int n, const double* in, double* out, doubl
   double* rb = st ? ra + t->N : t->rho_b;
   double* va = st ? rb + t->N : t->va;
   double* vbf = st ? va + t->N : t->vb_buf;
-  const int fl = t->flags & (QBH_ALL | QBH_FP32);
+  const int fl = whole_h_flags(t->flags);
   if (int r = potential(t, in, ra, va)) return r;
   h->v = va;
   if (int r = set_A(t, &t->A[3 * n])) return r;
@@ -575,7 +575,7 @@ extern "C" int qbh_td_backward(qbh_td t, double* dvext_dev, double* dA) {
   const size_t B = t->blk();
   const int M = t->nsteps, o = t->order;
   const long nc = (long)t->ngw * t->nst;
-  const int fl = t->flags & (QBH_ALL | QBH_FP32);
+  const int fl = whole_h_flags(t->flags);
   const double hdt = 0.5 * t->dt;
   k_fill<<<nb256(3L * (M + 1)), 256, 0, c->stream>>>(t->gA, 3L * (M + 1));
   k_fill<<<nb256(t->N), 256, 0, c->stream>>>(t->gv, t->N);

@@ -158,7 +158,7 @@ extern "C" int qbh_ham_autotune(qbh_ham ham, const double* psi_dev, long ld, int
   Ham* h = &ham->h;
   Ctx* c = h->ctx;
   hipSetDevice(c->device);
-  const int fl = (flags & (QBH_ALL | QBH_FP32)) == 0 ? QBH_ALL : (flags & (QBH_ALL | QBH_FP32));
+  const int fl = whole_h_flags(flags);
   const int mask = knob_mask == 0 ? (1 << NKNOB) - 1 : knob_mask;
   const std::string key = shape_key(h, nst, fl) + "." + std::to_string(mask);
   std::string rep;
@@ -317,7 +317,7 @@ extern "C" int qbh_batch_create(qbh_ctx ctx, int ncells, const qbh_ham* hams,
   hipSetDevice(uc->device);
   auto* bt = new qbh_batch_s();
   bt->user = uc;
-  bt->flags = (flags & (QBH_ALL | QBH_FP32)) == 0 ? QBH_ALL : (flags & (QBH_ALL | QBH_FP32));
+  bt->flags = whole_h_flags(flags);
   // cells of one Hamiltonian share its scratch: keep them on one branch
   std::map<Ham*, int> ham_branch;
   nbranch = std::max(1, std::min(nbranch, ncells));

@@ -161,3 +161,21 @@ def test_mixed_precision_scf_energy(gpu_ctx):
     print("dE/atom", de, "density L1 per electron", l1)
     assert de < 1e-6 and l1 < 1e-5
     _close(s)
+
+
+def test_scf_flags_without_terms_mean_the_whole_hamiltonian(gpu_ctx):
+    """flags_inner = QBH_FP32 and flags_last = 0 (no term named) run the same SCF as
+    QBH_ALL|QBH_FP32 and QBH_ALL: the loop applies the whole H, never an empty one."""
+    from qball_amd import QBH_ALL, QBH_FP32
+    s = _setup(gpu_ctx, "g", 8, 45)
+    sysm = s["sysm"]
+    occ = np.full(sysm.nst, 2.0)
+    hists = []
+    for fi, fl in ((QBH_ALL | QBH_FP32, QBH_ALL), (QBH_FP32, 0)):
+        cd = to_dev(s["psi"])
+        hists.append(s["ham"].scf_run(s["vb"], cd.data_ptr(), s["basis"].ngw, sysm.nst, occ,
+                                      sysm.omega, s["vd"].data_ptr(), s["pd"].data_ptr(), 2, 2,
+                                      sysm.ecut, flags_inner=fi, flags_last=fl))
+    assert np.max(np.abs(hists[0] - hists[1])) <= 1e-12 * np.max(np.abs(hists[0]))
+    assert np.all(np.abs(hists[0][:, :, 1]) > 0)        # kinetic energy is there
+    _close(s)
