@@ -180,6 +180,25 @@ def test_density(gpu_ctx, which, request):
     assert rel_err(rho.cpu().numpy(), ref) <= FP64_TOL
 
 
+@pytest.mark.parametrize("which", ["si8", "si8c"])
+def test_density_fp32(gpu_ctx, which, request):
+    """ρ from the FP32 transforms (FP64 band sum) against the oracle at the FP32 bar."""
+    import torch
+    from oracle import hpsi as oh
+    from qball_amd import QBH_FP32
+    sysm = request.getfixturevalue(which)
+    ob, maps, _ = oracle_for(sysm)
+    basis = make_gpu_basis(gpu_ctx, sysm)
+    cd = to_dev(sysm.psi)
+    rho = torch.zeros(basis.N, dtype=torch.float64, device="cuda")
+    basis.density(cd.data_ptr(), basis.ngw, sysm.nst, sysm.occ, 1.0, sysm.omega, rho.data_ptr(),
+                  flags=QBH_FP32)
+    gpu_ctx.sync()
+    ref = oh.compute_density(maps, sysm.psi, sysm.occ, 1.0, sysm.omega)
+    basis.close()
+    assert rel_err(rho.cpu().numpy(), ref) <= FP32_TOL
+
+
 def test_density_rhog_si8(gpu_ctx, si8):
     """update_density's charge_vft: rhog = forward(Omega*rho) on the density basis."""
     import torch
@@ -421,6 +440,23 @@ def test_etrs_step_selfconsistent_si8c(gpu_ctx, si8c):
         got = from_dev(cd, basis.ngw, sysm.nst)
         assert rel_err(got, ref) <= FP64_TOL, step
         assert all(abs(a - b) <= 1e-10 for a, b in zip(e, eref)), (step, e, eref)
+    # the FP32-bulk step (QBH_FP32 alone selects the whole H) from the same start, against
+    # the FP64 result of the first oracle step
+    from qball_amd import QBH_ALL, QBH_FP32
+    ref1 = oracle_step(psi, sysm.v)[0]
+    outs = []
+    for fl in (QBH_FP32, QBH_ALL | QBH_FP32):
+        v0 = real_to_dev(sysm.v)
+        ham.set_vloc(v0.data_ptr())
+        gpu_ctx.sync()
+        c32 = to_dev(psi)
+        ham.etrs_step(vb, c32.data_ptr(), basis.ngw, sysm.nst, sysm.occ, 1.0, dt, vd.data_ptr(),
+                      pd.data_ptr(), sysm.omega, flags=fl)
+        gpu_ctx.sync()
+        outs.append(from_dev(c32, basis.ngw, sysm.nst))
+    print("FP32 ETRS step vs FP64 oracle", rel_err(outs[0], ref1))
+    assert np.array_equal(outs[0], outs[1])
+    assert rel_err(outs[0], ref1) <= 1e-5
     ham.close()
     basis.close()
     vb.close()
