@@ -527,3 +527,34 @@ def test_hpsi_spd_projectors(gpu_ctx, which, request):
     basis.close()
     assert rel_err(from_dev(hd, basis.ngw, sysm.nst), ref) <= FP64_TOL
     assert abs(enl - enl_ref) <= 1e-10
+
+
+@pytest.mark.parametrize("which", ["si8", "si8c"])
+@pytest.mark.parametrize("nb", [1, 2, 3])
+def test_few_bands(gpu_ctx, which, nb, request):
+    """One to three bands (at Γ one band is an odd last band alone in its transform): Hψ in
+    FP64 and FP32, the density and the non-local forces against the oracle."""
+    import torch
+    from oracle import hpsi as oh
+    sysm = request.getfixturevalue(which)
+    ob, maps, sp = oracle_for(sysm)
+    c = np.ascontiguousarray(sysm.psi[:, :nb])
+    occ = sysm.occ[:nb]
+    basis = make_gpu_basis(gpu_ctx, sysm)
+    ham = make_gpu_ham(gpu_ctx, basis, sysm)
+    cd = to_dev(c)
+    ref, enl_ref = oh.hpsi(maps, c, sysm.v, [sp], occ, sysm.omega)
+    for flags, tol in ((0x7, FP64_TOL), (0x17, FP32_TOL)):
+        hd = torch.full_like(cd, float("nan"))
+        ham.hpsi(cd.data_ptr(), basis.ngw, nb, occ, hd.data_ptr(), basis.ngw, flags=flags)
+        gpu_ctx.sync()
+        assert rel_err(from_dev(hd, basis.ngw, nb), ref) <= tol, flags
+    rho = torch.zeros(basis.N, dtype=torch.float64, device="cuda")
+    basis.density(cd.data_ptr(), basis.ngw, nb, occ, 1.0, sysm.omega, rho.data_ptr())
+    gpu_ctx.sync()
+    assert rel_err(rho.cpu().numpy(), oh.compute_density(maps, c, occ, 1.0, sysm.omega)) <= FP64_TOL
+    f, enl = ham.nl_forces(cd.data_ptr(), basis.ngw, nb, occ, want_enl=True)
+    enl_f, fref = oh.nonlocal_forces(ob, [sp], c, occ, sysm.omega)
+    assert rel_err(f, fref[0]) <= FP64_TOL and abs(enl - enl_ref) <= 1e-10
+    ham.close()
+    basis.close()
