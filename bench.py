@@ -47,6 +47,8 @@ def parse():
     p.add_argument("--no-batch", action="store_true", help="skip the 12-cell batch section")
     p.add_argument("--batch-reps", type=int, default=10)
     p.add_argument("--no-large", action="store_true", help="skip the Al500 large-cell section")
+    p.add_argument("--no-td", action="store_true",
+                   help="skip the Si64 TDDFT forward/adjoint section (differentiation form)")
     p.add_argument("--batch-streams", type=int, default=4,
                    help="contexts (HIP streams) the 12 batch cells are spread over")
     p.add_argument("--no-tune", action="store_true",
@@ -324,6 +326,10 @@ def main():
     if not args.no_large:
         large = large_cell_bench(args)
         log("large-cell section done")
+    td = None
+    if not args.no_td:
+        td = td_bench(args)
+        log("differentiation-form section done")
 
     # CPU baseline: the oracle (numpy restatement) on a bounded sample, 1 thread
     cpu = None
@@ -353,6 +359,7 @@ def main():
         "fp32_bulk": fp32,
         "batching": batch,
         "large_cell": large,
+        "differentiation": td,
         "tuning": sched64,
         "kernels_sequential_builtin": per_kernel,
         "band_energy_Ha": e64,
@@ -523,6 +530,66 @@ def large_cell_bench(args):
     out["subspace_diag_fp64_ms"] = (time.perf_counter() - t) * 1e3
     out["eig_min_max_Ha"] = [float(eig.min()), float(eig.max())]
     ham.close()
+    basis.close()
+    ctx.close()
+    return out
+
+
+def td_bench(args):
+    """The differentiation form: real-time TDDFT on Si64 in a complex basis (18277 plane
+    waves, 128 bands, 72^3 grid), 200 ETRS steps of dt = 0.05 under A(t) = A0 sin(0.3 t)
+    exp(-(t - t0)^2 / 2 tau^2) along x (A0 = 0.01), objective the time-integrated dipole
+    along x, psi stored every 20 steps: forward and adjoint (backward, with the segment
+    recomputation) times in FP64 and FP32-bulk, and the FP32-bulk J and dJ/dA0 against FP64."""
+    from qball_amd import QBH_FP32, Basis, Context, Hamiltonian, TDPropagator, capi
+    from qball_amd.devmem import DeviceArray, synchronize
+    from qball_amd.workload import dipole_weights, field_shape, make_system, td_ionic_inputs
+    nsteps, dt, ckpt, A0 = 200, 0.05, 20, 0.01
+    s = make_system(ncell=2, ecut_ry=25.0, nbands=128, seed=41, complex_basis=True)
+    ctx = Context(0, None)
+    r = s.rods
+    basis = Basis(ctx, False, r["rod_h"], r["rod_k"], r["rod_lmin"], r["rod_size"], s.np, s.b)
+    ham = Hamiltonian(ctx, basis)
+    v = DeviceArray.from_numpy(s.v)
+    ham.set_vloc(v.ptr)
+    ham.set_projectors([s.na], [len(s.lproj)], s.lproj, s.wt, s.twnl.ravel(), s.tau.ravel(),
+                       s.omega)
+    vr = capi.rods_build(s.a, 4.0 * s.ecut, False)
+    vb = Basis(ctx, False, vr["rod_h"], vr["rod_k"], vr["rod_lmin"], vr["rod_size"], s.np, s.b)
+    vion, rhops = td_ionic_inputs(s, vr)
+    dvion = DeviceArray.from_numpy(np.ascontiguousarray(vion).view(np.float64))
+    drhops = DeviceArray.from_numpy(np.ascontiguousarray(rhops).view(np.float64))
+    psi = DeviceArray.from_numpy(np.ascontiguousarray(s.psi.T).view(np.float64).ravel())
+    basis.gram(psi.ptr, s.ngw, s.nst)
+    n0, n1, n2 = s.np
+    z, y, x = np.meshgrid(np.arange(n2), np.arange(n1), np.arange(n0), indexing="ij")
+    vext = DeviceArray.from_numpy((0.02 * np.cos(2 * np.pi * x / n0) * np.cos(2 * np.pi * y / n1)
+                                   + 0.01 * np.sin(2 * np.pi * z / n2)).ravel())
+    gv = DeviceArray(8 * n0 * n1 * n2)
+    occ = np.full(s.nst, 2.0)
+    td = TDPropagator(ham, vb, s.nst, occ, 1.0, s.omega, dvion.ptr, drhops.ptr)
+    shape = field_shape(nsteps, dt, omega=0.3)
+    w = dipole_weights(nsteps, dt)
+    out = {"workload": f"Si{s.na} complex basis, {s.ngw} PWs, {s.nst} bands, grid {n0}^3, "
+                       f"{nsteps} ETRS steps of dt={dt}, checkpoint every {ckpt} steps"}
+    for name, fl in (("fp64", 0), ("fp32_bulk", QBH_FP32)):
+        synchronize()
+        t0 = time.perf_counter()
+        J = td.forward(psi.ptr, s.ngw, nsteps, dt, A0 * shape, w, vext_ptr=vext.ptr,
+                       checkpoint=ckpt, flags=fl)
+        synchronize()
+        t1 = time.perf_counter()
+        gA = td.backward(gv.ptr)
+        synchronize()
+        t2 = time.perf_counter()
+        out[name] = {"J": J, "dJ_dA0": float(gA[:, 0] @ shape[:, 0]), "forward_s": t1 - t0,
+                     "backward_s": t2 - t1, "forward_steps_per_s": nsteps / (t1 - t0)}
+    out["fp32_bulk"]["J_rel_vs_fp64"] = abs(out["fp32_bulk"]["J"] - out["fp64"]["J"]) / abs(out["fp64"]["J"])
+    out["fp32_bulk"]["dJ_dA0_rel_vs_fp64"] = (abs(out["fp32_bulk"]["dJ_dA0"] - out["fp64"]["dJ_dA0"])
+                                              / abs(out["fp64"]["dJ_dA0"]))
+    td.close()
+    ham.close()
+    vb.close()
     basis.close()
     ctx.close()
     return out

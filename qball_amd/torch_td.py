@@ -6,35 +6,15 @@ propagator (``TDPropagator``), differentiable in ``vext`` (device float64 grid) 
 The extension module must have been built (``make -C qball_amd/csrc torch``): there is no
 fallback.
 """
-import math
-
 import numpy as np
 import torch
+
+from .workload import dipole_weights, field_shape  # noqa: F401  (part of this module's API)
 
 try:
     from . import _qbh_torch
 except ImportError as exc:  # the product path has no substitute
     raise ImportError("qball_amd/_qbh_torch*.so is not built: run make -C qball_amd/csrc torch") from exc
-
-
-def field_shape(nsteps, dt, omega=0.3, t0=None, tau=None, direction=(1.0, 0.0, 0.0)):
-    """Time profile of the uniform velocity-gauge field A(t) = A0 sin(omega t)
-    exp(-(t - t0)^2 / (2 tau^2)) along `direction` at t_n = n dt, n = 0..nsteps; returns
-    (nsteps+1, 3) so that A_n = A0 * shape_n."""
-    T = nsteps * dt
-    t0 = 0.5 * T if t0 is None else t0
-    tau = T / 6.0 if tau is None else tau
-    t = dt * np.arange(nsteps + 1)
-    s = np.sin(omega * t) * np.exp(-((t - t0) ** 2) / (2.0 * tau * tau))
-    d = np.asarray(direction, dtype=np.float64)
-    return np.outer(s, d / math.sqrt(float(d @ d)))
-
-
-def dipole_weights(nsteps, dt):
-    """Weights w_n for J = integral_0^T d(t) dt with d(t_n) = sum_{m<n} dt j(t_m) (the
-    dipole accumulated from the total current, rectangle rule): w_m = dt^2 (nsteps - m)."""
-    m = np.arange(nsteps + 1)
-    return dt * dt * (nsteps - m).astype(np.float64)
 
 
 def td_objective(prop, psi0_ptr, ld, vext, A0, nsteps, dt, shape, w, e=(1.0, 0.0, 0.0), order=4,

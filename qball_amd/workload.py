@@ -239,3 +239,40 @@ def random_psi(ngw, nst, g2, rng, real=True):
     if real:
         c[0, :] = c[0, :].real
     return c
+
+
+# --------------------------------------------------------------------------------------
+# real-time TDDFT inputs (the differentiation form)
+def field_shape(nsteps, dt, omega=0.3, t0=None, tau=None, direction=(1.0, 0.0, 0.0)):
+    """Time profile of the uniform velocity-gauge field A(t) = A0 sin(omega t)
+    exp(-(t - t0)^2 / (2 tau^2)) along `direction` at t_n = n dt, n = 0..nsteps; returns
+    (nsteps+1, 3) so that A_n = A0 * shape_n."""
+    T = nsteps * dt
+    t0 = 0.5 * T if t0 is None else t0
+    tau = T / 6.0 if tau is None else tau
+    t = dt * np.arange(nsteps + 1)
+    s = np.sin(omega * t) * np.exp(-((t - t0) ** 2) / (2.0 * tau * tau))
+    d = np.asarray(direction, dtype=np.float64)
+    return np.outer(s, d / math.sqrt(float(d @ d)))
+
+
+def dipole_weights(nsteps, dt):
+    """Weights w_n for J = integral_0^T d(t) dt with d(t_n) = sum_{m<n} dt j(t_m) (the
+    dipole accumulated from the total current, rectangle rule): w_m = dt^2 (nsteps - m)."""
+    m = np.arange(nsteps + 1)
+    return dt * dt * (nsteps - m).astype(np.float64)
+
+
+def td_ionic_inputs(sysm, vrods, valence=4.0, vloc=-10.0):
+    """Ionic inputs of physical size on the density basis with rods `vrods` (its G order):
+    Gaussian ion charges rho_ps(G) = -(valence/Omega) S(G) exp(-G^2/4), neutral with
+    valence * natoms electrons, and a short-range local potential v_ion(G) = (vloc/Omega)
+    S(G) exp(-G^2/2), S(G) = sum_a exp(-i G.tau_a).  Both stay a few Ha in real space for
+    any cell size, unlike per-G inputs of fixed size, whose Hartree potential grows with the
+    cell.  Returns (vion, rhops), complex per G."""
+    _, kpg = plane_waves(vrods, sysm.b)
+    g2 = (kpg ** 2).sum(axis=1)
+    sg = np.exp(-1j * (kpg @ sysm.tau.T)).sum(axis=1)
+    rhops = -(valence / sysm.omega) * sg * np.exp(-0.25 * g2)
+    vion = (vloc / sysm.omega) * sg * np.exp(-0.5 * g2)
+    return vion, rhops

@@ -23,7 +23,7 @@ def si64td():
     import torch
     from oracle.basis import Basis as OB, UnitCell
     from qball_amd import Basis, Context, TDPropagator, capi
-    from qball_amd.workload import make_system
+    from qball_amd.workload import make_system, td_ionic_inputs
     sysm = make_system(ncell=2, ecut_ry=25.0, nbands=128, seed=41, complex_basis=True)
     stream = torch.cuda.current_stream()
     ctx = Context(0, stream.cuda_stream)
@@ -33,12 +33,9 @@ def si64td():
     vb = Basis(ctx, False, r["rod_h"], r["rod_k"], r["rod_lmin"], r["rod_size"], sysm.np, sysm.b)
     vob = OB(UnitCell(sysm.a), 4.0 * sysm.ecut, force_complex=True)
     assert vob.size == vb.ngw
-    # ionic inputs of physical size: Gaussian ion charges of valence 4 (neutral with the 256
-    # electrons) and a short-range local potential, both on the structure factor S(G), so
-    # that |v(r)| stays at a few Ha and dt * |H| inside the Taylor propagator's range
-    sg = np.exp(-1j * (vob.kpgx @ sysm.tau.T)).sum(axis=1)
-    rhops = -(4.0 / sysm.omega) * sg * np.exp(-0.25 * vob.g2)
-    vion = (-10.0 / sysm.omega) * sg * np.exp(-0.5 * vob.g2)
+    # ionic inputs of physical size (Gaussian ion charges neutral with the 256 electrons and a
+    # short-range local potential), so that dt * |H| stays inside the Taylor propagator's range
+    vion, rhops = td_ionic_inputs(sysm, r)
     cplx = lambda z: torch.from_numpy(np.ascontiguousarray(z).view(np.float64).copy()).cuda()
     psi = cplx(np.ascontiguousarray(sysm.psi.T))
     basis.gram(psi.data_ptr(), basis.ngw, sysm.nst)        # orthonormal start (FP64, GPU)
