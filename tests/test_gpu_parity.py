@@ -558,3 +558,49 @@ def test_few_bands(gpu_ctx, which, nb, request):
     assert rel_err(f, fref[0]) <= FP64_TOL and abs(enl - enl_ref) <= 1e-10
     ham.close()
     basis.close()
+
+
+@pytest.mark.parametrize("which", ["si8", "si8c"])
+def test_padded_leading_dimensions(gpu_ctx, which, request):
+    """qball's ComplexMatrix has mloc >= ngw rows per band (zero padding rows): ψ with
+    ld = ngw + 3 and Hψ with ldh = ngw + 7.  Hψ in FP64 and FP32, cleared (padding rows of Hψ
+    come out zero) and accumulating (padding rows untouched); density and Gram on the same
+    padded ψ."""
+    import torch
+    from oracle import hpsi as oh
+    sysm = request.getfixturevalue(which)
+    ob, maps, sp = oracle_for(sysm)
+    basis = make_gpu_basis(gpu_ctx, sysm)
+    ham = make_gpu_ham(gpu_ctx, basis, sysm)
+    ngw, nst = basis.ngw, sysm.nst
+    ld, ldh = ngw + 3, ngw + 7
+    pad = np.zeros((ld, nst), dtype=np.complex128)
+    pad[:ngw] = sysm.psi
+    cd = to_dev(pad)
+    ref = oh.hpsi(maps, sysm.psi, sysm.v, [sp], sysm.occ, sysm.omega)[0]
+    for flags, tol in ((0x7, FP64_TOL), (0x17, FP32_TOL)):
+        hd = to_dev(np.full((ldh, nst), 5.0 + 3.0j))
+        ham.hpsi(cd.data_ptr(), ld, nst, sysm.occ, hd.data_ptr(), ldh, flags=flags)
+        gpu_ctx.sync()
+        h = hd.cpu().numpy().view(np.complex128).reshape(nst, ldh).T
+        assert rel_err(h[:ngw], ref) <= tol, flags
+        assert np.all(h[ngw:] == 0), flags                      # cleared output
+        start = np.full((ldh, nst), 1.0 - 2.0j)
+        hd = to_dev(start)
+        ham.hpsi(cd.data_ptr(), ld, nst, sysm.occ, hd.data_ptr(), ldh, flags=flags | 0x8)
+        gpu_ctx.sync()
+        h = hd.cpu().numpy().view(np.complex128).reshape(nst, ldh).T
+        assert rel_err(h[:ngw] - start[:ngw], ref) <= tol, flags
+        assert np.all(h[ngw:] == start[ngw:]), flags            # accumulate: untouched
+    rho = torch.zeros(basis.N, dtype=torch.float64, device="cuda")
+    basis.density(cd.data_ptr(), ld, nst, sysm.occ, 1.0, sysm.omega, rho.data_ptr())
+    gpu_ctx.sync()
+    assert rel_err(rho.cpu().numpy(), oh.compute_density(maps, sysm.psi, sysm.occ, 1.0,
+                                                         sysm.omega)) <= FP64_TOL
+    basis.gram(cd.data_ptr(), ld, nst)
+    gpu_ctx.sync()
+    g = cd.cpu().numpy().view(np.complex128).reshape(nst, ld).T
+    assert rel_err(g[:ngw], oh.gram(sysm.psi, sysm.real)) <= FP64_TOL
+    assert np.all(g[ngw:] == 0)
+    ham.close()
+    basis.close()
