@@ -58,8 +58,9 @@ std::string b64_encode(const unsigned char* p, size_t n) {
   return out;
 }
 
-// decodes base64, skipping white space; returns bytes written
-size_t b64_decode(const char* s, size_t n, unsigned char* out) {
+// decodes base64, skipping white space, into at most `cap` bytes; returns the bytes
+// written, or cap + 1 when the text holds more than cap bytes (nothing past cap is written)
+size_t b64_decode(const char* s, size_t n, unsigned char* out, size_t cap) {
   int val[256];
   for (int i = 0; i < 256; ++i) val[i] = -1;
   for (int i = 0; i < 64; ++i) val[(unsigned char)kB64[i]] = i;
@@ -78,6 +79,8 @@ size_t b64_decode(const char* s, size_t n, unsigned char* out) {
     }
     if (nq == 4) {
       const unsigned v = (q[0] << 18) | (q[1] << 12) | (q[2] << 6) | q[3];
+      const size_t k = pad >= 2 ? 1 : pad == 1 ? 2 : 3;
+      if (o + k > cap) return cap + 1;
       out[o++] = (v >> 16) & 255;
       if (pad < 2) out[o++] = (v >> 8) & 255;
       if (pad < 1) out[o++] = v & 255;
@@ -262,6 +265,11 @@ extern "C" int qbh_sd_read_xml(qbh_basis basis, const char* path, long offset, d
       break;
     }
     const size_t gt = s.find('>', g0);
+    const size_t g1 = gt == std::string::npos ? gt : s.find("</grid_function>", gt);
+    if (g1 == std::string::npos || g1 > sd1) {
+      r = iofail(c, QBH_ERR_ARG, "unterminated <grid_function>");
+      break;
+    }
     const std::string tag = s.substr(g0, gt - g0);
     std::ostringstream want;
     want << "nx=\"" << g.np0 << "\" ny=\"" << g.np1 << "\" nz=\"" << g.np2 << "\"";
@@ -270,9 +278,9 @@ extern "C" int qbh_sd_read_xml(qbh_basis basis, const char* path, long offset, d
       r = iofail(c, QBH_ERR_UNSUPPORTED, "grid_function does not match the basis grid / type");
       break;
     }
-    const size_t g1 = s.find("</grid_function>", gt);
+    const size_t cap = vals.size() * sizeof(double);
     const size_t nb = b64_decode(s.data() + gt + 1, g1 - gt - 1,
-                                 reinterpret_cast<unsigned char*>(vals.data()));
+                                 reinterpret_cast<unsigned char*>(vals.data()), cap);
     if (nb != vals.size() * sizeof(double)) {
       r = iofail(c, QBH_ERR_ARG, "grid_function size differs from the basis grid");
       break;

@@ -77,3 +77,35 @@ def test_xml_slater_determinant(gpu_ctx, tmp_path, complex_basis):
     assert nxt == len(text.encode()) and np.array_equal(occ2, occ)
     assert rel_err(from_dev(back, basis.ngw, s.nst), s.psi) <= 1e-13
     basis.close()
+
+
+def test_xml_reader_rejects_malformed_grid_functions(gpu_ctx, tmp_path):
+    """A grid function whose base64 text holds more data than the basis grid, or one with no
+    closing tag, is an error (the decoder never writes past the grid buffer and never reads
+    past the file); the well-formed file still reads back."""
+    import torch
+    from qball_amd import Basis, QbhError, capi
+    from qball_amd.workload import make_system
+    s = make_system(ncell=1, ecut_ry=25.0, nbands=2, seed=3)
+    r = s.rods
+    grid = tuple(int(x) for x in r["np"])
+    basis = Basis(gpu_ctx, s.real, r["rod_h"], r["rod_k"], r["rod_lmin"], r["rod_size"], grid, s.b)
+    cd = to_dev(s.psi)
+    p = tmp_path / "sd.xml"
+    capi.sd_write_xml(basis, p, cd.data_ptr(), basis.ngw, s.nst, s.occ)
+    text = p.read_text()
+    back = torch.zeros_like(cd)
+    # extra base64 quanta ahead of the first grid function's data: more bytes than the grid
+    m = re.search(r"(<grid_function[^>]*>\n)", text)
+    longer = text[:m.end(1)] + "AAAA" * 3000 + "\n" + text[m.end(1):]
+    (tmp_path / "long.xml").write_text(longer)
+    with pytest.raises(QbhError, match="size differs"):
+        capi.sd_read_xml(basis, tmp_path / "long.xml", back.data_ptr(), basis.ngw, s.nst)
+    # no closing tag on any grid function
+    (tmp_path / "cut.xml").write_text(text.replace("</grid_function>", ""))
+    with pytest.raises(QbhError, match="unterminated"):
+        capi.sd_read_xml(basis, tmp_path / "cut.xml", back.data_ptr(), basis.ngw, s.nst)
+    capi.sd_read_xml(basis, p, back.data_ptr(), basis.ngw, s.nst)
+    gpu_ctx.sync()
+    assert rel_err(from_dev(back, basis.ngw, s.nst), s.psi) <= 1e-13
+    basis.close()
