@@ -510,7 +510,8 @@ k_xpass(GridDev g, cplx<T>* __restrict__ t2, int nslots, int slots_per_wg,
 //   stage B  (bwd): DFT_N2 over contiguous blocks -> psi(r) at x = k1 + N1*k2, in
 //                   registers; VMUL multiplies by v(r) there and runs stage B of the
 //                   forward transform (DFT_N2, twiddle W_n^{-k1 m2}) without leaving
-//                   registers; DENSITY accumulates |psi|^2 there
+//                   registers; DENSITY accumulates |psi|^2 there; STORE puts psi(r)
+//                   back into the rows in natural order and writes the rows to the grid
 //   stage A' (fwd): DFT_N1 over stride-N2 elements -> natural order
 // Three LDS round trips per slot instead of one per radix pass, and the x-dependent
 // multiply touches each element once, in registers.
@@ -676,6 +677,44 @@ k_xpass2(GridDev g, cplx<T>* __restrict__ t2, int nslots, int slots_per_wg,
       }
     }
     wave_sync();
+    if constexpr (OP == X_STORE) {
+      // stage B backward for every block of the wavefront's rows first (in registers), then
+      // psi(r) back into the rows in natural order and out to the slot's grid: each
+      // wavefront's rows are one contiguous stretch of the grid (z fixed, y consecutive)
+      cplx<T> bs[IB][N2];
+#pragma unroll
+      for (int ib = 0; ib < IB; ++ib) {
+        const int t = lane + 64 * ib;
+        if (t < TB) {
+          const int r = t / N1, k1 = t % N1;
+          const cplx<T>* blk = buf + r * LD + S * k1;
+#pragma unroll
+          for (int j2 = 0; j2 < N2; ++j2) bs[ib][j2] = blk[j2];
+          sdft<N2, +1>(bs[ib]);
+        }
+      }
+      wave_sync();
+#pragma unroll
+      for (int ib = 0; ib < IB; ++ib) {
+        const int t = lane + 64 * ib;
+        if (t < TB) {
+          const int r = t / N1, k1 = t % N1;
+#pragma unroll
+          for (int k2 = 0; k2 < N2; ++k2) buf[r * LD + PD::idx(k1 + N1 * k2)] = bs[ib][k2];
+        }
+      }
+      wave_sync();
+      const long ngrid = (long)n * np1 * np2;
+      for (int e = lane; e < nyl * n; e += 64) {
+        const int r = e / n, x = e - r * n;
+        const cplx<T> q = buf[r * LD + PD::idx(x)];
+        double* fp = out + 2 * ((long)s * ngrid + grow0 + e);
+        fp[0] = double(q.x);
+        fp[1] = double(q.y);
+      }
+      __syncthreads();  // the next scatter overwrites the rows read above
+      return;
+    }
     // stage B: backward DFT_N2 -> psi(r); multiply / accumulate; forward DFT_N2
     double f1 = 0, f2 = 0, b1 = 0, b2 = 0;
     if (OP == X_DENSITY) {
@@ -1130,7 +1169,7 @@ bool register_edge(int n) { return has_x2(n); }
 template <typename T, int OP>
 static bool launch_x2(Ctx* ctx, const GridDev& g, dim3 grid, size_t lds, cplx<T>* t2, int nslots,
                       int spw, const double* in, double* out, const double* dens, int cd) {
-  if constexpr (OP != X_VMUL && OP != X_DENSITY) {
+  if constexpr (OP != X_VMUL && OP != X_DENSITY && OP != X_STORE) {
     return false;
   } else {
   // line elements per thread: a short count for spheres up to about half the edge (the
@@ -1166,7 +1205,8 @@ static int xpass_t(Ctx* ctx, Basis* b, const PipelineArgs& a, const double* in, 
                    const double* dens, double scale, int cplx_density, const char* name) {
   const GridDev& g = b->g;
   auto* t2 = static_cast<cplx<T>*>(a.t2);
-  const bool reg = (OP == X_VMUL || OP == X_DENSITY) && has_x2(g.np0) && b->tune.x_reg;
+  const bool reg = (OP == X_VMUL || OP == X_DENSITY || OP == X_STORE) && has_x2(g.np0) &&
+                   b->tune.x_reg;
   const int rows = reg ? Cfg<T>::XR : Cfg<T>::LPW;  // rows per wavefront
   const int yb = rows * NWAVE;
   const int nyb = (g.np1 + yb - 1) / yb;
