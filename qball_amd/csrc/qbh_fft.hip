@@ -992,7 +992,12 @@ static void reg_factors(int n, int& f1, int& f2) {
   switch (n) {
     case 28: f1 = 7; f2 = 4; break;
     case 36: f1 = 6; f2 = 6; break;
+    case 60: f1 = 10; f2 = 6; break;
+    case 64: f1 = 8; f2 = 8; break;
     case 72: f1 = 9; f2 = 8; break;
+    case 90: f1 = 10; f2 = 9; break;
+    case 96: f1 = 12; f2 = 8; break;
+    case 120: f1 = 12; f2 = 10; break;
     case 140: f1 = 14; f2 = 10; break;
     case 144: f1 = 12; f2 = 12; break;
     default: break;
@@ -1051,8 +1056,28 @@ static int check_launch(Ctx* ctx, const char* what) {
         constexpr int NM = 144, F1 = 6, F2 = 6;          \
         CALL;                                            \
       } break;                                           \
+      case 60: {                                         \
+        constexpr int NM = 144, F1 = 10, F2 = 6;         \
+        CALL;                                            \
+      } break;                                           \
+      case 64: {                                         \
+        constexpr int NM = 144, F1 = 8, F2 = 8;          \
+        CALL;                                            \
+      } break;                                           \
       case 72: {                                         \
         constexpr int NM = 144, F1 = 9, F2 = 8;          \
+        CALL;                                            \
+      } break;                                           \
+      case 90: {                                         \
+        constexpr int NM = 144, F1 = 10, F2 = 9;         \
+        CALL;                                            \
+      } break;                                           \
+      case 96: {                                         \
+        constexpr int NM = 144, F1 = 12, F2 = 8;         \
+        CALL;                                            \
+      } break;                                           \
+      case 120: {                                        \
+        constexpr int NM = 144, F1 = 12, F2 = 10;        \
         CALL;                                            \
       } break;                                           \
       case 140: {                                        \
@@ -1190,7 +1215,12 @@ static bool launch_x2(Ctx* ctx, const GridDev& g, dim3 grid, size_t lds, cplx<T>
   switch (g.np0) {
     QBH_X2(28, 7, 4)
     QBH_X2(36, 6, 6)
+    QBH_X2(60, 10, 6)
+    QBH_X2(64, 8, 8)
     QBH_X2(72, 9, 8)
+    QBH_X2(90, 10, 9)
+    QBH_X2(96, 12, 8)
+    QBH_X2(120, 12, 10)
     QBH_X2(140, 14, 10)
     QBH_X2(144, 12, 12)
     default:

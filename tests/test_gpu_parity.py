@@ -277,7 +277,8 @@ def test_vector_potential_complex_si8(gpu_ctx, si8c):
     assert rel_err(got, ref) <= FP64_TOL
 
 
-@pytest.mark.parametrize("grid", [(40, 30, 42), (28, 36, 72), (30, 33, 44)])
+@pytest.mark.parametrize("grid", [(40, 30, 42), (28, 36, 72), (30, 33, 44), (60, 64, 90),
+                                  (64, 90, 96), (90, 96, 120), (96, 120, 60), (120, 60, 64)])
 def test_noncubic_grids_generic_and_register_paths(gpu_ctx, si8, grid):
     """Non-cubic grids mixing edges with and without a register factorisation: transforms
     and local Hψ through both the LDS Stockham and the two-factor register passes."""
@@ -312,6 +313,9 @@ def test_noncubic_grids_generic_and_register_paths(gpu_ctx, si8, grid):
     gpu_ctx.sync()
     ref = oh.rs_mul_add(maps, si8.psi, v, np.zeros_like(si8.psi))
     assert rel_err(from_dev(hd, basis.ngw, nb), ref) <= FP64_TOL
+    ham.hpsi(cd.data_ptr(), basis.ngw, nb, si8.occ, hd.data_ptr(), basis.ngw, flags=0x14)
+    gpu_ctx.sync()
+    assert rel_err(from_dev(hd, basis.ngw, nb), ref) <= FP32_TOL
     rho = torch.zeros(N, dtype=torch.float64, device="cuda")
     basis.density(cd.data_ptr(), basis.ngw, nb, si8.occ, 1.0, si8.omega, rho.data_ptr())
     gpu_ctx.sync()
