@@ -991,13 +991,22 @@ static void reg_factors(int n, int& f1, int& f2) {
   f1 = f2 = 0;
   switch (n) {
     case 28: f1 = 7; f2 = 4; break;
+    case 32: f1 = 8; f2 = 4; break;
     case 36: f1 = 6; f2 = 6; break;
+    case 40: f1 = 8; f2 = 5; break;
+    case 48: f1 = 8; f2 = 6; break;
+    case 50: f1 = 10; f2 = 5; break;
+    case 54: f1 = 9; f2 = 6; break;
     case 60: f1 = 10; f2 = 6; break;
     case 64: f1 = 8; f2 = 8; break;
     case 72: f1 = 9; f2 = 8; break;
+    case 80: f1 = 10; f2 = 8; break;
     case 90: f1 = 10; f2 = 9; break;
     case 96: f1 = 12; f2 = 8; break;
+    case 100: f1 = 10; f2 = 10; break;
+    case 108: f1 = 12; f2 = 9; break;
     case 120: f1 = 12; f2 = 10; break;
+    case 128: f1 = 16; f2 = 8; break;
     case 140: f1 = 14; f2 = 10; break;
     case 144: f1 = 12; f2 = 12; break;
     default: break;
@@ -1052,6 +1061,42 @@ static int check_launch(Ctx* ctx, const char* what) {
         constexpr int NM = 144, F1 = 7, F2 = 4;          \
         CALL;                                            \
       } break;                                           \
+      case 32: {                                   \
+        constexpr int NM = 144, F1 = 8, F2 = 4;    \
+        CALL;                                      \
+      } break;                                     \
+      case 40: {                                   \
+        constexpr int NM = 144, F1 = 8, F2 = 5;    \
+        CALL;                                      \
+      } break;                                     \
+      case 48: {                                   \
+        constexpr int NM = 144, F1 = 8, F2 = 6;    \
+        CALL;                                      \
+      } break;                                     \
+      case 50: {                                   \
+        constexpr int NM = 144, F1 = 10, F2 = 5;   \
+        CALL;                                      \
+      } break;                                     \
+      case 54: {                                   \
+        constexpr int NM = 144, F1 = 9, F2 = 6;    \
+        CALL;                                      \
+      } break;                                     \
+      case 80: {                                   \
+        constexpr int NM = 144, F1 = 10, F2 = 8;   \
+        CALL;                                      \
+      } break;                                     \
+      case 100: {                                  \
+        constexpr int NM = 144, F1 = 10, F2 = 10;  \
+        CALL;                                      \
+      } break;                                     \
+      case 108: {                                  \
+        constexpr int NM = 144, F1 = 12, F2 = 9;   \
+        CALL;                                      \
+      } break;                                     \
+      case 128: {                                  \
+        constexpr int NM = 144, F1 = 16, F2 = 8;   \
+        CALL;                                      \
+      } break;                                     \
       case 36: {                                         \
         constexpr int NM = 144, F1 = 6, F2 = 6;          \
         CALL;                                            \
@@ -1214,6 +1259,15 @@ static bool launch_x2(Ctx* ctx, const GridDev& g, dim3 grid, size_t lds, cplx<T>
   }
   switch (g.np0) {
     QBH_X2(28, 7, 4)
+    QBH_X2(32, 8, 4)
+    QBH_X2(40, 8, 5)
+    QBH_X2(48, 8, 6)
+    QBH_X2(50, 10, 5)
+    QBH_X2(54, 9, 6)
+    QBH_X2(80, 10, 8)
+    QBH_X2(100, 10, 10)
+    QBH_X2(108, 12, 9)
+    QBH_X2(128, 16, 8)
     QBH_X2(36, 6, 6)
     QBH_X2(60, 10, 6)
     QBH_X2(64, 8, 8)
