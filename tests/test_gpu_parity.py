@@ -235,21 +235,24 @@ def test_gram(gpu_ctx, which, request):
     assert rel_err(got, ref) <= 1e-11
 
 
-def test_exponential_complex_si8(gpu_ctx, si8c):
+@pytest.mark.parametrize("order", [1, 2, 4, 6, 8])
+def test_exponential_complex_si8(gpu_ctx, si8c, order):
+    """The Taylor exponential at every order the entry point takes the ends and middle of
+    (ExponentialWavefunctionStepper.cc:51-148 uses 4)."""
     from oracle import hpsi as oh
     ob, maps, sp = oracle_for(si8c)
     basis = make_gpu_basis(gpu_ctx, si8c)
     ham = make_gpu_ham(gpu_ctx, basis, si8c)
     cd = to_dev(si8c.psi)
     dt = 0.05
-    ham.exponential(cd.data_ptr(), basis.ngw, si8c.nst, dt)
+    ham.exponential(cd.data_ptr(), basis.ngw, si8c.nst, dt, order)
     gpu_ctx.sync()
     got = from_dev(cd, basis.ngw, si8c.nst)
     occ0 = np.zeros(si8c.nst)
 
     def apply_h(w):
         return oh.hpsi(maps, w, si8c.v, [sp], occ0, si8c.omega)[0]
-    ref = oh.exponential(apply_h, si8c.psi, dt)
+    ref = oh.exponential(apply_h, si8c.psi, dt, order)
     ham.close()
     basis.close()
     assert rel_err(got, ref) <= FP64_TOL
@@ -272,9 +275,13 @@ def test_vector_potential_complex_si8(gpu_ctx, si8c):
     got = from_dev(hd, basis.ngw, si8c.nst)
     ref = oh.hpsi(maps, si8c.psi, si8c.v, [sp], si8c.occ, si8c.omega,
                   kpg2=oh.kpgpa2(ob, A), kpgx=ob.kpgx - A[None, :])[0]
+    assert rel_err(got, ref) <= FP64_TOL
+    # the FP32 form builds its own FP32 projector table at k+G-A
+    ham.hpsi(cd.data_ptr(), basis.ngw, si8c.nst, si8c.occ, hd.data_ptr(), basis.ngw, flags=0x17)
+    gpu_ctx.sync()
+    assert rel_err(from_dev(hd, basis.ngw, si8c.nst), ref) <= FP32_TOL
     ham.close()
     basis.close()
-    assert rel_err(got, ref) <= FP64_TOL
 
 
 _REG_EDGES = (32, 40, 48, 50, 54, 80, 100, 108, 128)
