@@ -95,8 +95,19 @@ def test_nl_forces_two_species_all_l(gpu_ctx, which, request):
     basis, ham = _ham(gpu_ctx, sysm, [s1, s2])
     cd = to_dev(sysm.psi)
     f, enl = ham.nl_forces(cd.data_ptr(), basis.ngw, sysm.nst, occ, want_enl=True)
+    # the Hψ of the same two species (projector columns of both in one product)
+    import torch
+    from helpers import from_dev
+    from oracle import hpsi as oh
+    hd = torch.zeros_like(cd)
+    ham.hpsi(cd.data_ptr(), basis.ngw, sysm.nst, occ, hd.data_ptr(), basis.ngw)
+    gpu_ctx.sync()
+    got = from_dev(hd, basis.ngw, sysm.nst)
     ham.close()
     basis.close()
+    ob, maps, _ = oracle_for(sysm)
+    href = oh.hpsi(maps, sysm.psi, sysm.v, [s1, s2], occ, sysm.omega)[0]
+    assert rel_err(got, href) <= FP64_TOL
     enl_ref, ref = _oracle(sysm, [s1, s2], occ)
     assert f.shape == (sysm.na, 3)
     assert rel_err(f, ref) <= FP64_TOL
