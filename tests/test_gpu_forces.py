@@ -98,7 +98,6 @@ def test_nl_forces_two_species_all_l(gpu_ctx, which, request):
     # the Hψ of the same two species (projector columns of both in one product)
     import torch
     from helpers import from_dev
-    from oracle import hpsi as oh
     hd = torch.zeros_like(cd)
     ham.hpsi(cd.data_ptr(), basis.ngw, sysm.nst, occ, hd.data_ptr(), basis.ngw)
     gpu_ctx.sync()
