@@ -1009,6 +1009,14 @@ static void reg_factors(int n, int& f1, int& f2) {
     case 128: f1 = 16; f2 = 8; break;
     case 140: f1 = 14; f2 = 10; break;
     case 144: f1 = 12; f2 = 12; break;
+    case 160: f1 = 16; f2 = 10; break;
+    case 168: f1 = 14; f2 = 12; break;
+    case 180: f1 = 15; f2 = 12; break;
+    case 192: f1 = 16; f2 = 12; break;
+    case 210: f1 = 15; f2 = 14; break;
+    case 224: f1 = 16; f2 = 14; break;
+    case 240: f1 = 16; f2 = 15; break;
+    case 256: f1 = 16; f2 = 16; break;
     default: break;
   }
 }
@@ -1133,6 +1141,38 @@ static int check_launch(Ctx* ctx, const char* what) {
         constexpr int NM = 144, F1 = 12, F2 = 12;        \
         CALL;                                            \
       } break;                                           \
+      case 168: {                                  \
+        constexpr int NM = 144, F1 = 14, F2 = 12;  \
+        CALL;                                      \
+      } break;                                     \
+      case 210: {                                  \
+        constexpr int NM = 144, F1 = 15, F2 = 14;  \
+        CALL;                                      \
+      } break;                                     \
+      case 224: {                                  \
+        constexpr int NM = 144, F1 = 16, F2 = 14;  \
+        CALL;                                      \
+      } break;                                     \
+      case 160: {                                  \
+        constexpr int NM = 144, F1 = 16, F2 = 10;  \
+        CALL;                                      \
+      } break;                                     \
+      case 180: {                                  \
+        constexpr int NM = 144, F1 = 15, F2 = 12;  \
+        CALL;                                      \
+      } break;                                     \
+      case 192: {                                  \
+        constexpr int NM = 144, F1 = 16, F2 = 12;  \
+        CALL;                                      \
+      } break;                                     \
+      case 240: {                                  \
+        constexpr int NM = 144, F1 = 16, F2 = 15;  \
+        CALL;                                      \
+      } break;                                     \
+      case 256: {                                  \
+        constexpr int NM = 144, F1 = 16, F2 = 16;  \
+        CALL;                                      \
+      } break;                                     \
       default:                                           \
         if ((n) <= 144) {                                \
           constexpr int NM = 144, F1 = 0, F2 = 0;        \
@@ -1277,6 +1317,14 @@ static bool launch_x2(Ctx* ctx, const GridDev& g, dim3 grid, size_t lds, cplx<T>
     QBH_X2(120, 12, 10)
     QBH_X2(140, 14, 10)
     QBH_X2(144, 12, 12)
+    QBH_X2(160, 16, 10)
+    QBH_X2(168, 14, 12)
+    QBH_X2(210, 15, 14)
+    QBH_X2(224, 16, 14)
+    QBH_X2(180, 15, 12)
+    QBH_X2(192, 16, 12)
+    QBH_X2(240, 16, 15)
+    QBH_X2(256, 16, 16)
     default:
       return false;
   }

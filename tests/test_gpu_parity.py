@@ -291,7 +291,9 @@ _REG_EDGES = (32, 40, 48, 50, 54, 80, 100, 108, 128)
                                   (64, 90, 96), (90, 96, 120), (96, 120, 60), (120, 60, 64)]
                          + [(_REG_EDGES[i], _REG_EDGES[(i + 3) % 9], _REG_EDGES[(i + 6) % 9])
                             for i in range(9)]
-                         + [(150, 160, 192)])             # edges past 144: Stockham, NMAX 256
+                         + [(150, 162, 196)]              # edges past 144 on the Stockham passes
+                         + [g for L in (160, 168, 180, 192, 210, 224, 240, 256)
+                            for g in ((L, 36, 40), (36, L, 40), (36, 40, L))])
 def test_noncubic_grids_generic_and_register_paths(gpu_ctx, si8, grid):
     """Non-cubic grids mixing edges with and without a register factorisation: transforms
     and local Hψ through both the LDS Stockham and the two-factor register passes."""
