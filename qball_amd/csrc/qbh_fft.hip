@@ -1258,12 +1258,14 @@ int pipeline_fwd_zy_accumulate(Ctx* ctx, Basis* b, const PipelineArgs& a, double
                              : fwd_zy_t<float>(ctx, b, a, out, ldo, kin, store);
 }
 
-// slots per x-pass workgroup: keep >= 8192 workgroups in flight while amortising the
-// potential rows and twiddles over several slots
+// slots per x-pass workgroup: keep >= 4096 workgroups in flight while amortising the
+// potential rows and twiddles over several slots.  Measured on the Si64 complex basis (72^3,
+// 128 slots): 4 slots per workgroup (11520 workgroups) 0.760/0.516 ms per FP64/FP32 Hψ, 8
+// (5760) 0.747/0.495 ms, 16 (2880) 0.754/0.484 ms; Si512 and Al500 reach the cap of 16.
 static int x_slots_per_wg(const GridDev& g, int nslots, int yb) {
   const long rows = (long)g.np2 * ((g.np1 + yb - 1) / yb);
   int spw = 1;
-  while (spw < 16 && rows * ((nslots + 2 * spw - 1) / (2 * spw)) >= 8192) spw *= 2;
+  while (spw < 16 && rows * ((nslots + 2 * spw - 1) / (2 * spw)) >= 4096) spw *= 2;
   return spw;
 }
 
