@@ -292,3 +292,33 @@ def test_etrs_step_splits_and_order():
     assert np.max(np.abs(got - ref)) <= bound
     wrong = expm(H1, 0.5 * dt) @ expm(H2, 0.5 * dt) @ psi
     assert np.max(np.abs(got - wrong)) > 100 * bound
+
+
+@pytest.mark.parametrize("real", [False, True])
+def test_s_projector_sits_at_its_atom(real):
+    """beta_s(r) = sum_G anl(G) e^{iG.r} (direct sum, no transform) with a positive radial
+    table is real and largest at r = tau: the structure-factor phase e^{-iG.tau} of comp_eigr
+    (NonLocalPotential.cc:1969, kpgr = -(k+G).tau) puts the projector on the atom.  A phase
+    e^{+iG.tau} would put it at -tau instead."""
+    cell = UnitCell(np.array([[6.0, 0.2, 0.0], [0.0, 5.5, 0.4], [0.3, 0.0, 6.5]]))
+    b = Basis(cell, 5.0, force_complex=not real)
+    tau = np.array([[1.3, 2.1, 0.7]])
+    tw = np.exp(-0.3 * b.kpg2)[None, :]
+    sp = oh.Projectors([0], [1.0], tw, tau)
+    a = oh.anl_block(sp, b.kpgx, 0, 1)[:, 0]
+    g = b.kpgx
+    if real:  # the half sphere stands for G and -G: beta(-G) = conj(beta(G)), G=0 once
+        g = np.concatenate([g, -g[1:]])
+        a = np.concatenate([a, np.conj(a[1:])])
+
+    def beta(r):
+        return np.sum(a * np.exp(1j * (g @ r)))
+
+    rng = np.random.default_rng(1)
+    at = beta(tau[0])
+    assert abs(at.imag) < 1e-9 * abs(at) and abs(at.real - np.sum(np.exp(-0.3 * (g * g).sum(1)))) < 1e-9 * abs(at)
+    for _ in range(20):
+        d = rng.normal(size=3)
+        d *= rng.uniform(0.2, 2.0) / np.linalg.norm(d)
+        assert abs(beta(tau[0] + d)) < at.real
+    assert abs(beta(-tau[0])) < 0.5 * at.real

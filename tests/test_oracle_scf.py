@@ -115,3 +115,17 @@ def test_scf_reaches_a_self_consistent_fixed_point():
     wk, wls = mixing_weights(vb)
     assert wk[0] == 0.0 and wls[0] == 1.0
     assert kinetic_energy(wb, c, occ) > 0.0
+
+
+def test_mixing_weights_half_points():
+    """Kerker weight g2/(g2 + q0^2) is 1/2 where |G| = q0 = 2 pi / rcut and the row weight
+    sqrt(g2/(g2 + q1^2)) is 1/sqrt(2) where |G| = q1 = 2 pi / rc1 (BOSampleStepper.cc:425-444);
+    a q0 = 1/rcut slip moves both half points."""
+    from types import SimpleNamespace
+    q0, q1 = 2.0 * math.pi / 10.0, 2.0 * math.pi / 3.0
+    vb = SimpleNamespace(g2=np.array([0.0, q0 * q0, q1 * q1, 1e6]))
+    wk, wls = mixing_weights(vb, rc_kerker=10.0, rc1=3.0)
+    assert abs(wk[1] - 0.5) < 1e-15 and abs(wls[2] - math.sqrt(0.5)) < 1e-15
+    assert wk[0] == 0.0 and wls[0] == 1.0 and abs(wk[3] - 1.0) < 1e-5
+    wk0, _ = mixing_weights(vb, rc_kerker=0.0)
+    assert np.all(wk0 == 1.0)

@@ -72,3 +72,40 @@ def test_vhxc_energies_by_real_space_routes(real):
     # v_r = v_xc + v_ion + v_H on the grid
     assert np.max(np.abs(vr - (pz81(rho)[1] + vion_r.real + vh_r))) <= 1e-12 * np.max(np.abs(vr))
     assert abs(exc - dv * np.sum(rho * pz81(rho)[0])) <= 1e-13 * abs(exc)
+
+
+def _ec(rs):
+    """Correlation part of the oracle: e_xc minus the Slater exchange -(3/4)(3/(2 pi))^(2/3)/rs."""
+    rs = np.asarray(rs, dtype=np.float64)
+    rho = 3.0 / (4.0 * math.pi * rs ** 3)
+    ex = -0.75 * (9.0 / (4.0 * math.pi * math.pi)) ** (1.0 / 3.0) / rs
+    return pz81(rho)[0] - ex
+
+
+def test_pz81_high_density_constants_against_published_values():
+    """rs < 1: e_c = A ln rs + B + C rs ln rs + D rs.  The constants are read back from the
+    oracle's output and compared with numbers that do not come from it: A with the exact
+    Gell-Mann--Brueckner coefficient (1 - ln 2)/pi^2 (PZ81 quotes it to three digits), B, C
+    and D with the published values of Perdew & Zunger, Phys. Rev. B 23, 5048 (1981),
+    Table XII (unpolarized): B = -0.0480, C = 0.0020, D = -0.0116 (C and D are rounded
+    there; the oracle takes them from continuity at rs = 1)."""
+    # the four basis functions fit the branch exactly: solve at four radii below 1
+    rs = np.array([0.01, 0.05, 0.2, 0.6])
+    M = np.stack([np.log(rs), np.ones(4), rs * np.log(rs), rs], axis=1)
+    A, B, C, D = np.linalg.solve(M, _ec(rs))
+    assert abs(A - (1.0 - math.log(2.0)) / math.pi ** 2) < 1e-4
+    assert abs(A - 0.0311) < 1e-9 and abs(B - (-0.0480)) < 1e-9
+    assert abs(C - 0.0020) < 1e-4 and abs(D - (-0.0116)) < 1e-4
+
+
+def test_pz81_low_density_constants_against_published_values():
+    """rs >= 1: 1/e_c = (1 + b1 sqrt(rs) + b2 rs)/gamma is linear in (1, sqrt(rs), rs); the
+    three coefficients read back from the oracle give gamma = -0.1423, beta1 = 1.0529,
+    beta2 = 0.3334 (PZ81 Table XII, unpolarized), and e_c(rs = 1) = gamma/(1+beta1+beta2)."""
+    rs = np.array([1.5, 4.0, 9.0])
+    M = np.stack([np.ones(3), np.sqrt(rs), rs], axis=1)
+    c0, c1, c2 = np.linalg.solve(M, 1.0 / _ec(rs))
+    gamma = 1.0 / c0
+    assert abs(gamma - (-0.1423)) < 1e-10
+    assert abs(c1 * gamma - 1.0529) < 1e-10 and abs(c2 * gamma - 0.3334) < 1e-10
+    assert abs(_ec(np.array([1.0]))[0] - (-0.1423 / (1.0 + 1.0529 + 0.3334))) < 1e-12
