@@ -258,11 +258,11 @@ int qbh_scf_run(qbh_ham ham, qbh_basis vbasis, double* psi_dev, long ld, int nst
  *   "radix"   0..2 Stockham radix plan variant for the other lengths (basis-wide)
  *   "nsplit"  0..64 K slices of the projector contraction, 0 = heuristic
  *   "overlap" -1/0/1 projector GEMMs concurrent with the transforms, -1 = context default
- *   "fnl_algo", "enl_algo"  -1..16 projector GEMMs F = anl^T psi / hpsi += anl F by
- *             rocBLAS (0) or by the n-th algorithm hipBLASLt lists for the shape (F over the
- *             full K; n past the end of the list: the last); -1 = built-in: 2 with
- *             hipBLASLt 1.2 or newer loaded, else 0 (qbh_ctx_library_info); complex bases,
- *             for which hipBLASLt lists none, use rocBLAS */
+ *   "fnl_algo", "enl_algo"  -1..17 projector GEMMs F = anl^T psi / hpsi += anl F by
+ *             this library's matrix-core kernels (17; F over K slices), by rocBLAS (0) or
+ *             by the n-th algorithm hipBLASLt lists for the shape (1..16; F over the full
+ *             K; n past the end of the list: the last); -1 = built-in: 17 for a Γ basis;
+ *             complex bases use rocBLAS whatever the value */
 int qbh_ham_set_tuning(qbh_ham ham, const char* knob, int value);
 int qbh_ham_get_tuning(qbh_ham ham, const char* knob, int* value);
 /* Times candidate knob values for this Hψ shape (reps applications each, HIP events on the

@@ -56,15 +56,18 @@ __global__ void k_kinetic(int ngw, int nst, const double* __restrict__ kin,
 }
 
 // ------------------------------------------------------------------------------------
-// chunked transform driver: calls body(args) for consecutive chunks of slots
+// chunked transform driver: calls body(args) for consecutive chunks of slots, with the
+// transform workspace `ws` of `cap` bytes (at least one slot), or with the context's
+// workspace (grown on demand up to its cap) when ws is null
 template <typename F>
-static int for_chunks(Ctx* c, Basis* b, int nslots_total, int prec, F&& body) {
+static int for_chunks(Ctx* c, Basis* b, int nslots_total, int prec, F&& body,
+                      void* ws = nullptr, size_t cap = 0) {
   if (nslots_total <= 0) return 0;
   const size_t per = slot_bytes(b, prec);
-  size_t S = std::max<size_t>(1, c->ws_cap / per);
+  size_t S = std::max<size_t>(1, (ws ? cap : c->ws_cap) / per);
   S = std::min<size_t>(S, (size_t)nslots_total);
   S = std::min<size_t>(S, 65535);
-  void* ws = ws_get(c, S * per);
+  if (!ws) ws = ws_get(c, S * per);
   if (!ws) return fail(c, QBH_ERR_ALLOC, "transform workspace allocation failed");
   const size_t csz = prec == PREC_FP64 ? 16 : 8;
   char* t1 = static_cast<char*>(ws);
@@ -316,6 +319,12 @@ int qbh::hpsi_impl(Ham* h, const double* psi, long ld, int nst, const double* oc
   const int ngw = b->g.ngw;
   if (ld < ngw || ldh < ngw) return fail(c, QBH_ERR_ARG, "leading dimension < ngw");
   if (psi == hpsi) return fail(c, QBH_ERR_ARG, "psi and hpsi must not alias");
+  // every entry point that applies H comes through here (Hψ, exponential, ETRS step, TD,
+  // SCF, autotuner, batch): complex elements are read as 16-byte vectors, and the
+  // conversion and kinetic kernels put the band index in gridDim.y
+  if ((reinterpret_cast<uintptr_t>(psi) | reinterpret_cast<uintptr_t>(hpsi)) & 15)
+    return fail(c, QBH_ERR_ARG, "psi/hpsi must be 16-byte aligned");
+  if (nst <= 0 || nst > 65535) return fail(c, QBH_ERR_ARG, "nst must be in 1..65535");
   const bool fp32 = (flags & QBH_FP32) != 0;
   const int prec = fp32 ? PREC_FP32 : PREC_FP64;
   if (enl) *enl = 0.0;

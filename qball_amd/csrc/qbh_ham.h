@@ -54,10 +54,11 @@ struct Ham {
   // tuning (qbh_ham_set_tuning / qbh_ham_autotune): 0 / -1 = built-in choice
   int nsplit = 0;    // K slices of the projector contraction
   int overlap = -1;  // projector GEMMs on the side stream (1), after the transforms (0)
-  // projector GEMM library: 0 = rocBLAS (F over nsplit K slices); n >= 1 = hipBLASLt with
+  // projector GEMM implementation: kOwnGemm = this library's matrix-core kernels (real Γ
+  // view; F over K slices); 0 = rocBLAS (F over nsplit K slices); 1..16 = hipBLASLt with
   // the n-th algorithm its heuristic lists for the shape (F over the full K; n past the end
-  // of the list: the last one); shapes hipBLASLt lists nothing for (complex) use rocBLAS;
-  // -1 = built-in choice (nl_builtin_algo)
+  // of the list: the last one); shapes hipBLASLt lists nothing for (complex) and complex
+  // bases asked for kOwnGemm use rocBLAS; -1 = built-in choice (nl_builtin_algo)
   int fnl_algo = -1, enl_algo = -1;
   void* etrs = nullptr;           // ETRS step: psi(t+dt/2), psi~(t+dt), rho, rhog
   size_t etrs_bytes = 0;
@@ -90,8 +91,11 @@ int nl_forces(Ham* h, const double* psi, long ld, int nst, const double* occ, in
 // number of hipBLASLt algorithms listed for the fnl (which 0) or enl (which 1) GEMM of an
 // Hψ call with these arguments (0: hipBLASLt does not cover it)
 int nl_lt_candidates(Ham* h, int which, int nst, long ld, long ldh, int fp32);
-// the algorithm value the built-in choice (-1) of fnl_algo / enl_algo stands for
-int nl_builtin_algo(Ctx* c);
+// fnl_algo / enl_algo value of the matrix-core kernels of qbh_gemm.hip
+constexpr int kOwnGemm = 17;
+// the value the built-in choice (-1) of fnl_algo / enl_algo stands for: kOwnGemm for the
+// real Γ view, rocBLAS (0) for complex bases
+int nl_builtin_algo(Ham* h);
 int blas_handle(Ctx* c, void** out);
 // frees a context's rocBLAS handle and hipBLASLt state (qbh_nl.hip)
 void blas_destroy(Ctx* c);

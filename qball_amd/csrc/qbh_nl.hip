@@ -8,8 +8,10 @@
 //
 // The projector table anl is generated once per geometry / vector potential and kept in
 // HBM (2*ngw*P doubles; 2.4 GB for 512 Si atoms), so an Hψ call streams it twice
-// through the two MFMA GEMMs: rocBLAS (dgemm/zgemm/sgemm/cgemm, F split over K slices) or,
-// for the real Γ view, hipBLASLt with a tuned algorithm over the full K.
+// through the two GEMMs.  For the real Γ view they are this library's own matrix-core
+// kernels (qbh_gemm.hip; F split over K slices); complex bases use rocBLAS
+// (zgemm/cgemm, F split over K slices).  rocBLAS and hipBLASLt (a tuned algorithm over the
+// full K) remain selectable for the Γ view as tuning choices.
 #include <hip/hip_runtime.h>
 #include <hipblaslt/hipblaslt.h>
 #include <rocblas/rocblas.h>
@@ -22,6 +24,7 @@
 #include <vector>
 
 #include "../../include/qbh.h"
+#include "qbh_gemm.h"
 #include "qbh_ham.h"
 
 namespace qbh {
@@ -109,10 +112,14 @@ static bool lt_ready(Ctx* c) {
 // products and the second was within 1% of the fastest; an older hipBLASLt in the same
 // process (as bundled with a PyTorch build for ROCm 7.0) listed 16x16-tile FP64 kernels
 // about 5x slower than rocBLAS for the same shapes.
-int nl_builtin_algo(Ctx* c) {
+int nl_lt_builtin_algo(Ctx* c) {
   if (!lt_ready(c)) return 0;
   return static_cast<LtState*>(c->lt)->version >= 100200 ? 2 : 0;
 }
+
+// Built-in choice of both projector products: the matrix-core kernels of qbh_gemm.hip for
+// the real Γ view, rocBLAS for complex bases.
+int nl_builtin_algo(Ham* h) { return h->b->g.real ? kOwnGemm : 0; }
 
 Ctx* ctx_of(qbh_ctx c);
 
@@ -338,6 +345,23 @@ static int blas_ok(Ctx* c, rocblas_status s, const char* what) {
   return QBH_ERR_BLAS;
 }
 
+// the context's rocBLAS handle on its current stream, created on first use (only the
+// rocBLAS choices and complex bases need one)
+static int blas_rocblas(Ctx* c, rocblas_handle* out) {
+  void* hb = nullptr;
+  if (int r = blas_handle(c, &hb)) {
+    c->last_error = "rocBLAS handle creation failed";
+    return r;
+  }
+  *out = static_cast<rocblas_handle>(hb);
+  return 0;
+}
+
+static int gemm_fail(Ctx* c, const char* what) {
+  c->last_error = std::string(what) + " product: invalid sizes for the matrix-core kernel";
+  return QBH_ERR_ARG;
+}
+
 static int ensure_anl(Ham* h, bool fp32) {
   Ctx* c = h->ctx;
   const int ngw = h->b->g.ngw;
@@ -509,17 +533,25 @@ static int fnl_stage(Ham* h, rocblas_handle bh, const double* psi, long ld, int 
   const long Kelem = real ? 2L * ngw : (long)ngw;  // contraction length in elements
   const int e = cplx ? 2 : 1;
   // hipBLASLt algorithm chosen for this shape (nullptr: rocBLAS)
-  o->sel = h->fnl_algo >= 0 ? h->fnl_algo : nl_builtin_algo(c);
-  LtPlan* fplan = o->sel > 0 ? nl_plan(h, 0, nst, ld, ldh, fp32) : nullptr;
+  o->sel = h->fnl_algo >= 0 ? h->fnl_algo : nl_builtin_algo(h);
+  if (o->sel == kOwnGemm && !real) o->sel = 0;  // complex bases: rocBLAS
+  const bool own = o->sel == kOwnGemm;
+  LtPlan* fplan = (o->sel > 0 && !own) ? nl_plan(h, 0, nst, ld, ldh, fp32) : nullptr;
   const int falgo = fplan ? std::min<int>(o->sel, (int)fplan->cand.size()) - 1 : -1;
   // K slices of kc elements; the slice count actually written (ceil(K / kc)) can be below
   // the requested split count, and the finish kernel sums exactly that many partials.
   // hipBLASLt contracts the full K in one product.
-  const int want = fplan ? 1 : (h->nsplit > 0 ? h->nsplit : fnl_splits(P, nst, Kelem));
-  const long kc = (Kelem + want - 1) / want;
+  // the own kernel cuts K into its own slices (multiples of its 16-row stage)
+  const int want = fplan ? 1
+                         : own ? mfma_slices(P, nst, Kelem, h->nsplit)
+                               : (h->nsplit > 0 ? h->nsplit : fnl_splits(P, nst, Kelem));
+  const long kc = own ? mfma_slice_len(Kelem, want) : (Kelem + want - 1) / want;
   const int nsplit = (int)((Kelem + kc - 1) / kc);
   const long pstride = (long)P * nst * e;
   const size_t fsz = (size_t)pstride * nsplit;
+  if (!own && !bh) {
+    if (int r = blas_rocblas(c, &bh)) return r;
+  }
   if (!fp32) {
     double* F = static_cast<double*>(grow(h->f64, h->f64_bytes, fsz * sizeof(double)));
     const double* anl = static_cast<const double*>(h->anl64);
@@ -529,10 +561,15 @@ static int fnl_stage(Ham* h, rocblas_handle bh, const double* psi, long ld, int 
     }
     {
       TimedLaunch tl(c, "fnl_gemm");
-      if (int r = fplan ? lt_gemm<double>(c, fplan, falgo, anl, psi, F, 0.0)
-                        : fnl_gemm<double>(c, bh, cplx, P, nst, Kelem, anl, Kelem, psi,
-                                           real ? 2 * ld : ld, F, kc))
+      if (own) {
+        if (mfma_gemm_qq(c->stream, false, P, nst, Kelem, anl, Kelem, psi, 2 * ld, F, P, nsplit,
+                         pstride))
+          return gemm_fail(c, "fnl");
+      } else if (int r = fplan ? lt_gemm<double>(c, fplan, falgo, anl, psi, F, 0.0)
+                               : fnl_gemm<double>(c, bh, cplx, P, nst, Kelem, anl, Kelem, psi,
+                                                  real ? 2 * ld : ld, F, kc)) {
         return r;
+      }
     }
     TimedLaunch tl(c, "fnl_finish");
     if (real)
@@ -558,10 +595,15 @@ static int fnl_stage(Ham* h, rocblas_handle bh, const double* psi, long ld, int 
     }
     {
       TimedLaunch tl(c, "fnl_gemm");
-      if (int r = fplan ? lt_gemm<float>(c, fplan, falgo, anl, p32, F, 0.0f)
-                        : fnl_gemm<float>(c, bh, cplx, P, nst, Kelem, anl, Kelem, p32, Kelem,
-                                          F, kc))
+      if (own) {
+        if (mfma_gemm_qq(c->stream, true, P, nst, Kelem, anl, Kelem, p32, Kelem, F, P, nsplit,
+                         pstride))
+          return gemm_fail(c, "fnl");
+      } else if (int r = fplan ? lt_gemm<float>(c, fplan, falgo, anl, p32, F, 0.0f)
+                               : fnl_gemm<float>(c, bh, cplx, P, nst, Kelem, anl, Kelem, p32,
+                                                 Kelem, F, kc)) {
         return r;
+      }
     }
     TimedLaunch tl(c, "fnl_finish");
     if (real)
@@ -587,24 +629,32 @@ int nl_apply(Ham* h, const double* psi, long ld, int nst, const double* occ_host
   const bool cplx = !real;
   const int P = h->P;
   if (int r = ensure_anl(h, fp32)) return r;
-  void* hb;
-  if (int r = blas_handle(c, &hb)) return r;
-  rocblas_handle bh = static_cast<rocblas_handle>(hb);
+  rocblas_handle bh = nullptr;
   const double* occ;
   if (int r = nl_occ(h, nst, occ_host, &occ)) return r;
   FnlStage fs;
-  if (int r = fnl_stage(h, bh, psi, ld, nst, ldh, fp32 != 0, occ, &fs)) return r;
+  if (int r = fnl_stage(h, nullptr, psi, ld, nst, ldh, fp32 != 0, occ, &fs)) return r;
 
   const long Kelem = real ? 2L * ngw : (long)ngw;
-  const int esel = h->enl_algo >= 0 ? h->enl_algo : nl_builtin_algo(c);
-  LtPlan* eplan = esel > 0 ? nl_plan(h, 1, nst, ld, ldh, fp32 != 0) : nullptr;
+  int esel = h->enl_algo >= 0 ? h->enl_algo : nl_builtin_algo(h);
+  if (esel == kOwnGemm && !real) esel = 0;  // complex bases: rocBLAS
+  const bool own = esel == kOwnGemm;
+  LtPlan* eplan = (esel > 0 && !own) ? nl_plan(h, 1, nst, ld, ldh, fp32 != 0) : nullptr;
   const int ealgo = eplan ? std::min<int>(esel, (int)eplan->cand.size()) - 1 : -1;
+  if (!own && !eplan) {
+    if (int r = blas_rocblas(c, &bh)) return r;
+  }
   if (!fp32) {
     const double* F = static_cast<const double*>(fs.F);
     const double* anl = static_cast<const double*>(h->anl64);
     TimedLaunch tl(c, "enl_gemm");
     const double beta = overwrite ? 0.0 : 1.0;
-    if (eplan) {
+    if (own) {
+      // hpsi[k, n] (+)= sum_p anl[k, p] F[p, n], taken transposed (r = band, c = row)
+      if (mfma_gemm_qc(c->stream, false, nst, (int)Kelem, P, F, P, anl, Kelem, hpsi, 2 * ldh,
+                       !overwrite))
+        return gemm_fail(c, "enl");
+    } else if (eplan) {
       if (int r = lt_gemm<double>(c, eplan, ealgo, anl, F, hpsi, beta)) return r;
     } else if (int r = blas_ok(c,
                                gemm_sb(bh, rocblas_operation_none, rocblas_operation_none,
@@ -624,7 +674,11 @@ int nl_apply(Ham* h, const double* psi, long ld, int nst, const double* occ_host
     }
     {
       TimedLaunch tl(c, "enl_gemm");
-      if (eplan) {
+      if (own) {
+        if (mfma_gemm_qc(c->stream, true, nst, (int)Kelem, P, F, P, anl, Kelem, t32, Kelem,
+                         false))
+          return gemm_fail(c, "enl");
+      } else if (eplan) {
         if (int r = lt_gemm<float>(c, eplan, ealgo, anl, F, t32, 0.0f)) return r;
       } else if (int r = blas_ok(c,
                                  gemm_sb(bh, rocblas_operation_none, rocblas_operation_none,
@@ -739,11 +793,17 @@ static int dfnl_forces(Ham* h, rocblas_handle bh, const double* psi, long ld, in
   const long Kelem = real ? 2L * ngw : (long)ngw;
   const int e = cplx ? 2 : 1;
   // psi' is compact (leading dimension ngw); same library choice as the fnl product
-  LtPlan* dplan = sel > 0 ? nl_plan(h, 0, nst, ngw, ngw, fp32) : nullptr;
+  const bool own = sel == kOwnGemm;
+  LtPlan* dplan = (sel > 0 && !own) ? nl_plan(h, 0, nst, ngw, ngw, fp32) : nullptr;
   const int dalgo = dplan ? std::min<int>(sel, (int)dplan->cand.size()) - 1 : -1;
-  const int want = dplan ? 1 : (h->nsplit > 0 ? h->nsplit : fnl_splits(P, nst, Kelem));
-  const long kc = (Kelem + want - 1) / want;
+  const int want = dplan ? 1
+                         : own ? mfma_slices(P, nst, Kelem, h->nsplit)
+                               : (h->nsplit > 0 ? h->nsplit : fnl_splits(P, nst, Kelem));
+  const long kc = own ? mfma_slice_len(Kelem, want) : (Kelem + want - 1) / want;
   const int nsplit = (int)((Kelem + kc - 1) / kc);
+  if (!own && !dplan && !bh) {
+    if (int r = blas_rocblas(c, &bh)) return r;
+  }
   const long pstride = (long)P * nst * e;
   T* dF = static_cast<T*>(grow(h->df, h->df_bytes, (size_t)pstride * nsplit * sizeof(T)));
   const int S = std::max(1, std::min(64, (nst + 15) / 16));
@@ -761,10 +821,15 @@ static int dfnl_forces(Ham* h, rocblas_handle bh, const double* psi, long ld, in
     }
     {
       TimedLaunch tl(c, "dfnl_gemm");
-      if (int r = dplan ? lt_gemm<T>(c, dplan, dalgo, anl, dpsi, dF, T(0))
-                        : fnl_gemm<T>(c, bh, cplx, P, nst, Kelem, anl, Kelem, dpsi, Kelem, dF,
-                                      kc))
+      if (own) {
+        if (mfma_gemm_qq(c->stream, fp32, P, nst, Kelem, anl, Kelem, dpsi, Kelem, dF, P, nsplit,
+                         pstride))
+          return gemm_fail(c, "dfnl");
+      } else if (int r = dplan ? lt_gemm<T>(c, dplan, dalgo, anl, dpsi, dF, T(0))
+                               : fnl_gemm<T>(c, bh, cplx, P, nst, Kelem, anl, Kelem, dpsi, Kelem,
+                                             dF, kc)) {
         return r;
+      }
     }
     TimedLaunch tl(c, "nl_force");
     const dim3 grid((unsigned)((P + 63) / 64), (unsigned)S);
@@ -790,13 +855,11 @@ int nl_forces(Ham* h, const double* psi, long ld, int nst, const double* occ_hos
   const int ngw = h->b->g.ngw;
   const int P = h->P;
   if (int r = ensure_anl(h, fp32)) return r;
-  void* hb;
-  if (int r = blas_handle(c, &hb)) return r;
-  rocblas_handle bh = static_cast<rocblas_handle>(hb);
+  rocblas_handle bh = nullptr;
   const double* occ;
   if (int r = nl_occ(h, nst, occ_host, &occ)) return r;
   FnlStage fs;
-  if (int r = fnl_stage(h, bh, psi, ld, nst, ngw, fp32 != 0, occ, &fs)) return r;
+  if (int r = fnl_stage(h, nullptr, psi, ld, nst, ngw, fp32 != 0, occ, &fs)) return r;
   double* g = static_cast<double*>(grow(h->fg, h->fg_bytes, 3 * (size_t)P * sizeof(double)));
   if (!g) {
     c->last_error = "force scratch allocation failed";
@@ -856,8 +919,9 @@ extern "C" int qbh_ctx_library_info(qbh_ctx ctx, char* buf, int cap) {
   int hv = 0;
   hipRuntimeGetVersion(&hv);
   const int lv = lt_ready(c) ? static_cast<LtState*>(c->lt)->version : 0;
-  std::snprintf(buf, (size_t)cap, "hip %d rocblas %s hipblaslt %d projector-gemm-builtin %s", hv,
-                rb, lv, nl_builtin_algo(c) ? "hipblaslt" : "rocblas");
+  std::snprintf(buf, (size_t)cap,
+                "hip %d rocblas %s hipblaslt %d projector-gemm-builtin mfma (complex bases: "
+                "rocblas)", hv, rb, lv);
   return QBH_OK;
 }
 

@@ -558,8 +558,11 @@ extern "C" int qbh_td_forward(qbh_td t, const double* psi0_dev, long ld, int nst
     *J = s;
     r = tcheck(c, "td forward");
   }
+  // restore the caller's potential and vector potential on every exit path, keeping the
+  // first error
   h->v = v_own;
-  if (r == QBH_OK) r = qbh_ham_set_vector_potential(t->hh, A_own);
+  const int ra = qbh_ham_set_vector_potential(t->hh, A_own);
+  if (r == QBH_OK) r = ra;
   if (r == QBH_OK) t->have_run = true;
   return r;
 }
@@ -648,7 +651,9 @@ extern "C" int qbh_td_backward(qbh_td t, double* dvext_dev, double* dA) {
   hipStreamSynchronize(c->stream);
   h->v = v_own;
   if (r == QBH_OK) r = tcheck(c, "td backward");
-  if (r == QBH_OK) r = qbh_ham_set_vector_potential(t->hh, A_own);
+  // restore the vector potential on every exit path, keeping the first error
+  const int ra = qbh_ham_set_vector_potential(t->hh, A_own);
+  if (r == QBH_OK) r = ra;
   if (r == QBH_OK && dA) {
     for (int n = 0; n <= M; ++n)
       for (int d = 0; d < 3; ++d) dA[3 * n + d] = ga[3 * n + d] + t->w[n] * t->e[d] * t->jhost[2 * n + 1];

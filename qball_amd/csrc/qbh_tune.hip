@@ -6,8 +6,8 @@
 //          (Stockham radix plan variant for lengths without a register transform);
 //   ham:   nsplit (K slices of the projector contraction, 0 = heuristic), overlap (projector
 //          GEMMs on a side stream concurrent with the transforms, -1 = context default),
-//          fnl_algo / enl_algo (projector GEMMs by rocBLAS (0) or by the n-th algorithm
-//          hipBLASLt lists for the shape (n >= 1)).
+//          fnl_algo / enl_algo (projector GEMMs by this library's matrix-core kernels (17),
+//          by rocBLAS (0) or by the n-th algorithm hipBLASLt lists for the shape (1..16)).
 // qbh_ham_autotune times candidate values with HIP events on the context's stream, one
 // knob at a time (coordinate descent from the current choice), and caches the result per
 // problem shape in the context.
@@ -95,7 +95,7 @@ static int set_knob(Ham* h, int k, int v) {
       h->overlap = v;
       return QBH_OK;
     default:
-      if (v < -1 || v > kMaxAlgo) return QBH_ERR_ARG;
+      if (v < -1 || (v > kMaxAlgo && v != kOwnGemm)) return QBH_ERR_ARG;
       (k == 6 ? h->fnl_algo : h->enl_algo) = v;
       return QBH_OK;
   }
@@ -184,11 +184,12 @@ extern "C" int qbh_ham_autotune(qbh_ham ham, const double* psi_dev, long ld, int
     for (int which = 0; nl && which < 2; ++which) {
       const int na = nl_lt_candidates(h, which, nst, ld, ldh, (fl & QBH_FP32) ? 1 : 0);
       std::vector<int> vals;
+      if (g.real) vals.push_back(kOwnGemm);  // the matrix-core kernels of this library
       for (int v = 0; v <= std::min(na, kMaxAlgo); ++v) vals.push_back(v);
-      if (na > 0) {
+      if (vals.size() > 1) {
         // start from the value the built-in choice stands for, so it is not timed twice
         int& knob = which ? h->enl_algo : h->fnl_algo;
-        if (knob < 0) knob = nl_builtin_algo(c);
+        if (knob < 0) knob = nl_builtin_algo(h);
         plan.push_back({6 + which, vals});
       }
     }

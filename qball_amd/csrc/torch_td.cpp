@@ -5,6 +5,7 @@
 // The backward re-runs the forward from the saved inputs first, so a forward call made on
 // the same propagator between this node's forward and backward cannot change the result;
 // checkpoints every `checkpoint` steps bound the memory of both passes.
+#include <torch/cuda.h>
 #include <torch/extension.h>
 
 #include <string>
@@ -28,6 +29,11 @@ struct Args {
 
 double run_forward(const Args& a, const torch::Tensor& vext, double A0, const torch::Tensor& shape,
                    const torch::Tensor& w, const torch::Tensor& e) {
+  // The propagator runs on its context's stream, which need not be torch's current one:
+  // wait for the device so that vext (produced on any torch stream) is complete before the
+  // context reads it.  qbh_td_forward / qbh_td_backward synchronise their own stream before
+  // returning, so their outputs are ready for torch afterwards.
+  torch::cuda::synchronize(vext.device().index());
   torch::Tensor A = (shape * A0).contiguous();
   double J = 0.0;
   const int r = qbh_td_forward(a.td, a.psi0, a.ld, a.nsteps, a.dt, a.order,

@@ -3,7 +3,8 @@
 A process that never imports torch, so libqbh.so runs on the HIP runtime, rocBLAS and
 hipBLASLt it was linked against (as in a C++ host), not on the copies bundled with torch.
 Checks Hψ of an 8-atom Si cell (Γ, and a complex basis) against the oracle for the built-in
-projector GEMM choice and for every algorithm hipBLASLt lists, in FP64 and in the FP32
+projector GEMM choice (the library's matrix-core kernels), rocBLAS, and every algorithm
+hipBLASLt lists, in FP64 and in the FP32
 form, into an output buffer full of NaN (which Hψ without accumulation must not read).
 Prints one JSON line with the errors.
 """
@@ -59,7 +60,7 @@ def main():
             out["cases"].append({"complex": complex_basis, "flags": flags, "knob": "builtin",
                                  "value": -1, "err": run(flags)})
             for knob in ("fnl_algo", "enl_algo"):
-                for val in range(0, 17):
+                for val in range(0, 18):
                     ham.set_tuning(knob, val)
                     out["cases"].append({"complex": complex_basis, "flags": flags, "knob": knob,
                                          "value": val, "err": run(flags)})

@@ -131,10 +131,12 @@ def test_nl_forces_vector_potential(gpu_ctx, which, request):
     assert rel_err(f, ref) <= FP64_TOL
 
 
-@pytest.mark.parametrize("algo,nsplit", [(0, 0), (0, 3), (1, 0), (2, 0), (16, 0)])
+@pytest.mark.parametrize("algo,nsplit", [(0, 0), (0, 3), (1, 0), (2, 0), (16, 0), (17, 0),
+                                         (17, 3)])
 def test_nl_forces_padded_ld_every_algorithm(gpu_ctx, si8, algo, nsplit):
-    """A leading dimension past ngw, an odd band count, rocBLAS over K slices (nsplit) and
-    hipBLASLt algorithms (fnl_algo) give the same forces."""
+    """A leading dimension past ngw, an odd band count, rocBLAS over K slices (nsplit),
+    hipBLASLt algorithms and the library's matrix-core kernels (fnl_algo) give the same
+    forces."""
     import torch
     _, _, sp = oracle_for(si8)
     occ = np.full(si8.nst, 2.0)

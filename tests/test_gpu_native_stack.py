@@ -1,6 +1,7 @@
 """Parity of Hψ on the system ROCm libraries (HIP runtime, rocBLAS, hipBLASLt as a C++ host
 links them), checked in a child process that never imports torch: every projector GEMM
-algorithm gives the oracle's Hψ, and the built-in choice is hipBLASLt there."""
+choice (the library's matrix-core kernels, rocBLAS and each hipBLASLt algorithm) gives the
+oracle's Hψ."""
 import json
 import os
 import subprocess
@@ -19,9 +20,9 @@ def test_hpsi_parity_on_system_rocm_libraries():
     out = json.loads(p.stdout.strip().splitlines()[-1])
     print(out["libraries"])
     assert "hipblaslt" in out["libraries"]
-    # this image's ROCm (hipBLASLt 1.2): the built-in projector GEMMs are hipBLASLt's
-    assert out["libraries"].endswith("projector-gemm-builtin hipblaslt"), out["libraries"]
-    assert len(out["cases"]) == 2 * 2 * (1 + 2 * 17)
+    # the built-in projector GEMMs are the library's own matrix-core kernels (Γ)
+    assert "projector-gemm-builtin mfma" in out["libraries"], out["libraries"]
+    assert len(out["cases"]) == 2 * 2 * (1 + 2 * 18)
     for c in out["cases"]:
         tol = 1e-12 if c["flags"] == 0x7 else 2e-6
         assert c["err"] <= tol, c
@@ -33,4 +34,4 @@ def test_library_info_in_a_torch_process(gpu_ctx):
     info = gpu_ctx.library_info()
     print(info)
     assert info.startswith("hip ") and " rocblas " in info and " hipblaslt " in info
-    assert info.split()[-1] in ("hipblaslt", "rocblas")
+    assert "projector-gemm-builtin mfma" in info

@@ -61,7 +61,10 @@ def test_every_knob_value_keeps_parity(gpu_ctx, which, grid):
              ("nsplit", 1), ("nsplit", 3), ("nsplit", 7), ("nsplit", 32), ("nsplit", 0),
              ("overlap", 0), ("overlap", 1), ("overlap", -1),
              ("fnl_algo", 1), ("enl_algo", 1), ("fnl_algo", 16), ("enl_algo", 16),
-             ("fnl_algo", 2), ("enl_algo", 2)]
+             ("fnl_algo", 2), ("enl_algo", 2),
+             # the library's matrix-core kernels, with their own K slices
+             ("fnl_algo", 17), ("enl_algo", 17), ("nsplit", 1), ("nsplit", 5), ("nsplit", 64),
+             ("nsplit", 0)]
     for knob, val in cases:
         ham.set_tuning(knob, val)
         assert ham.get_tuning(knob) == val
@@ -73,13 +76,14 @@ def test_every_knob_value_keeps_parity(gpu_ctx, which, grid):
 
 @pytest.mark.parametrize("flags,tol", [(0x7, TOL), (0x17, 2e-6)])
 def test_every_gemm_algorithm_keeps_parity(gpu_ctx, flags, tol):
-    """Each projector GEMM algorithm hipBLASLt lists for the shape (F over the full K)
-    gives the oracle's Hψ, in FP64 and in the FP32 form."""
+    """Each projector GEMM algorithm hipBLASLt lists for the shape (F over the full K), and
+    the library's own matrix-core kernels (17), give the oracle's Hψ, in FP64 and in the
+    FP32 form."""
     c = _cell(gpu_ctx, "si8", None, seed=3)
     ref = _ref(c)
     ham = c["ham"]
     for knob in ("fnl_algo", "enl_algo"):
-        for v in range(1, 17):
+        for v in range(1, 18):
             ham.set_tuning(knob, v)
             assert rel_err(_run(gpu_ctx, c, flags), ref) <= tol, (knob, v)
         ham.set_tuning(knob, 0)
