@@ -106,6 +106,15 @@ int qbh_fft_backward(qbh_basis basis, const double* c_dev, long ld, int nbands, 
                      double* f_dev, int flags);
 int qbh_fft_forward(qbh_basis basis, const double* f_dev, int nbands, int pair, double* c_dev,
                     long ld, int flags);
+/* qbh_fft_backward without any device allocation, for callers that cannot allocate (code
+ * under HIP graph capture, tight serving loops, harnesses that forbid hipMalloc).  No
+ * counterpart in the reference.  Same arguments and results as qbh_fft_backward; its
+ * transform workspace is a 256 MiB array each GPU holds for the life of the process, so a
+ * call fails (QBH_ERR_ARG) when one transform slot of the grid needs more (FP64: 16 bytes x
+ * (columns x np2 + np2 x x-lines x np1)); larger band counts run in chunks.  Calls that
+ * overlap on one GPU race on that array. */
+int qbh_fft_backward_noalloc(qbh_basis basis, const double* c_dev, long ld, int nbands, int pair,
+                             double* f_dev, int flags);
 /* forward of a real grid (ChargeDensity.cc:523-550): c = forward(scale * rho + 0i). */
 int qbh_fft_forward_real(qbh_basis basis, const double* rho_dev, double scale, double* c_dev,
                          int flags);

@@ -42,6 +42,7 @@ SIGNATURES = {
                                 _ip]),
     "qbh_factorizable": (_c_int, [_c_int]),
     "qbh_fft_backward": (_c_int, [_vp, _vp, _c_long, _c_int, _c_int, _vp, _c_int]),
+    "qbh_fft_backward_noalloc": (_c_int, [_vp, _vp, _c_long, _c_int, _c_int, _vp, _c_int]),
     "qbh_fft_forward": (_c_int, [_vp, _vp, _c_int, _c_int, _vp, _c_long, _c_int]),
     "qbh_fft_forward_real": (_c_int, [_vp, _vp, _c_double, _vp, _c_int]),
     "qbh_ham_create": (_c_int, [_vp, _vp, ctypes.POINTER(_vp)]),
@@ -229,6 +230,11 @@ class Basis:
     def backward(self, c_ptr, ld, nbands, pair, f_ptr, flags=0):
         self.ctx.check(lib().qbh_fft_backward(self.h, c_ptr, ld, nbands, int(pair), f_ptr, flags),
                        "qbh_fft_backward")
+
+    def backward_noalloc(self, c_ptr, ld, nbands, pair, f_ptr, flags=0):
+        """backward() through the entry point that allocates nothing (qbh_fft_backward_noalloc)."""
+        self.ctx.check(lib().qbh_fft_backward_noalloc(self.h, c_ptr, ld, nbands, int(pair), f_ptr,
+                                                      flags), "qbh_fft_backward_noalloc")
 
     def forward(self, f_ptr, nbands, pair, c_ptr, ld, flags=0):
         self.ctx.check(lib().qbh_fft_forward(self.h, f_ptr, nbands, int(pair), c_ptr, ld, flags),

@@ -6,6 +6,8 @@
 #include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <map>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -105,6 +107,75 @@ extern "C" int qbh_fft_backward(qbh_basis basis, const double* c_dev, long ld, i
     if (int r = pipeline_bwd_yz(c, b, a)) return r;
     return pipeline_x_store(c, b, a, f_dev + 2 * (long)s0 * N);
   });
+}
+
+// ------------------------------------------------------------------------------------
+// Allocation-free backward transform.  Its transform workspace (the z-pencil and y-line
+// tiles of each slot) is this file-scope array instead of memory the context allocates:
+// each GPU the library runs on holds one copy from module load until the process ends.
+// Host code never uses the array's name as a device address: qbh_ctx_create looks up this
+// device's address once (hipGetSymbolAddress, with the device current) and every call
+// passes that address.  Every call on one GPU shares the array and leaves its contents
+// behind; the transform passes write every workspace element they read, so nothing has to
+// start cleared.  Calls that overlap on one GPU (two streams, two threads) race.
+constexpr size_t kNoallocScratchBytes = size_t(256) << 20;  // 256 MiB
+static __device__ __align__(256) unsigned char scratch[kNoallocScratchBytes];
+
+namespace {
+std::mutex g_scratch_mu;
+std::map<int, void*> g_scratch_addr;  // device -> device address of scratch
+}  // namespace
+
+void qbh::noalloc_scratch_setup(Ctx* c) {
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  auto it = g_scratch_addr.find(c->device);
+  if (it == g_scratch_addr.end()) {
+    // the caller (qbh_ctx_create) has made c->device current
+    void* ptr = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !ptr) {
+      (void)hipGetLastError();
+      c->scratch = nullptr;
+      c->scratch_err = std::string("hipGetSymbolAddress of the no-allocation scratch failed: ") +
+                       hipGetErrorString(e);
+      return;  // not cached: a later context on this device tries again
+    }
+    it = g_scratch_addr.emplace(c->device, ptr).first;
+  }
+  c->scratch = it->second;
+}
+
+extern "C" int qbh_fft_backward_noalloc(qbh_basis basis, const double* c_dev, long ld,
+                                        int nbands, int pair, double* f_dev, int flags) {
+  if (!basis || !c_dev || !f_dev || nbands <= 0) return QBH_ERR_ARG;
+  Basis* b = basis_of(basis);
+  Ctx* c = ctx_of_basis(basis);
+  hipSetDevice(c->device);
+  if (ld < b->g.ngw) return fail(c, QBH_ERR_ARG, "ld < ngw");
+  if (pair && !b->g.real) return fail(c, QBH_ERR_ARG, "pair transforms need a real basis");
+  if (!c->scratch) return fail(c, QBH_ERR_DEVICE, c->scratch_err);
+  const int prec = (flags & QBH_FP32) ? PREC_FP32 : PREC_FP64;
+  const size_t per = slot_bytes(b, prec);
+  if (per > kNoallocScratchBytes)
+    return fail(c, QBH_ERR_ARG,
+                "one transform slot of this grid needs " + std::to_string(per) +
+                    " bytes, more than the " + std::to_string(kNoallocScratchBytes) +
+                    "-byte no-allocation scratch");
+  const int mode = pair ? SLOT_PAIR : (b->g.real ? SLOT_SINGLE_REAL : SLOT_COMPLEX);
+  const int nslots = pair ? (nbands + 1) / 2 : nbands;
+  const long N = (long)b->g.np0 * b->g.np1 * b->g.np2;
+  return for_chunks(
+      c, b, nslots, prec,
+      [&](PipelineArgs& a, int s0) {
+        a.psi = c_dev;
+        a.ld = ld;
+        a.nst = nbands;
+        a.band0 = pair ? 2 * s0 : s0;
+        a.mode = mode;
+        if (int r = pipeline_bwd_yz(c, b, a)) return r;
+        return pipeline_x_store(c, b, a, f_dev + 2 * (long)s0 * N);
+      },
+      c->scratch, kNoallocScratchBytes);
 }
 
 extern "C" int qbh_fft_forward(qbh_basis basis, const double* f_dev, int nbands, int pair,

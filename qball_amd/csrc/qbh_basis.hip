@@ -78,6 +78,7 @@ extern "C" int qbh_ctx_create(int device, void* stream, qbh_ctx* out) {
     }
     h->c.own_stream = true;
   }
+  qbh::noalloc_scratch_setup(&h->c);
   *out = h;
   return QBH_OK;
 }

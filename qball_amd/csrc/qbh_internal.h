@@ -108,7 +108,15 @@ struct Ctx {
   size_t ws_cap = size_t(8) << 30;  // cap on pipeline workspace
   // autotuner results per problem shape (key -> knob values)
   std::map<std::string, std::vector<int>> tune_cache;
+  // this device's allocation-free transform scratch (device address of the file-scope
+  // array in qbh_api.hip), or null with the reason in scratch_err
+  void* scratch = nullptr;
+  std::string scratch_err;
 };
+
+// Looks up (once per device, with `device` current) the device address of the
+// allocation-free scratch array and stores it in c->scratch (qbh_api.hip).
+void noalloc_scratch_setup(Ctx* c);
 
 // Records the named kernel's launch window when timing is enabled.
 struct TimedLaunch {
