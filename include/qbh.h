@@ -43,6 +43,7 @@ typedef struct qbh_td_s* qbh_td;
 typedef struct qbh_batch_s* qbh_batch;
 typedef struct qbh_psda_s* qbh_psda;
 typedef struct qbh_mixer_s* qbh_mixer;
+typedef struct qbh_species_s* qbh_species;
 
 /* ---- context ------------------------------------------------------------------------ */
 /* stream: a hipStream_t to run on, or NULL for a stream owned by the context. */
@@ -183,6 +184,48 @@ int qbh_subspace_diag(qbh_basis basis, double* psi_dev, long ld, const double* h
 int qbh_update_vhxc(qbh_basis vbasis, const double* rhor_dev, const double* rhog_dev,
                     const double* hamil_rhog_dev, const double* vion_g_dev,
                     const double* rhops_g_dev, double omega, double* v_r_dev, double* energies);
+
+/* ---- setup from a pseudopotential (Species, update_twnl, cell_moved / atoms_moved) ------
+ * A species holds the radial tables of Species::initialize for a norm-conserving
+ * Kleinman-Bylander pseudopotential (Species.cc:74-673): Vloc(G) of the local potential
+ * minus the long-range part of the Gaussian ion charge exp(-r^2/rcps^2), the KB weights
+ * wsg and the projector transforms, each on the reference's cubic spline in |G|.
+ * create: vps and phi hold (lmax+1) radial arrays of npts points on the mesh deltar (the
+ * row of llocal in phi is not read); create_xml reads them from a species XML file (the
+ * <norm_conserving_pseudopotential> element; semilocal species are QBH_ERR_UNSUPPORTED).
+ * Species need no context: the device tables are made on first use on a device. */
+int qbh_species_create(double zval, int lmax, int llocal, double deltar, int npts,
+                       const double* vps, const double* phi, double rcps, qbh_species* out);
+int qbh_species_create_xml(const char* path, double rcps, qbh_species* out);
+int qbh_species_destroy(qbh_species sp);
+/* zval, rcps, eself = zval^2 / (sqrt(2 pi) rcps) (Species.h:159-160), the number of KB
+ * projectors (sum of 2l+1 over l != llocal) and, per projector in the order of
+ * qbh_species_twnl, its l and weight wsg (each output nullable; lproj / wt sized nproj). */
+int qbh_species_info(qbh_species sp, double* zval, double* rcps, double* eself, int* nproj,
+                     int* lproj, double* wt);
+/* Host evaluation of the splines at n values g: which = -1 Vloc(G) (Species::dvlocg), else
+ * the projector of angular momentum l = which (Species::dvnlg); v, dv (nullable). */
+int qbh_species_eval(qbh_species sp, int which, int n, const double* g, double* v, double* dv);
+/* NonLocalPotential::update_twnl (NonLocalPotential.cc:261-1521, KB branch): twnl_dev
+ * (device, nproj rows of ngw) at |k+G-A| of the basis, A (nullable = 0) the vector
+ * potential, rows and angular factors in the reference's order (l ascending; p: x, y, z;
+ * d: y4..y8; f: y9..y15).  The rows, qbh_species_info's lproj and wt, and the atom
+ * positions are the arguments of qbh_ham_set_projectors. */
+int qbh_species_twnl(qbh_species sp, qbh_basis basis, const double* A, double* twnl_dev);
+/* EnergyFunctional::cell_moved + atoms_moved (EnergyFunctional.cc:1889-1915, 2001-2021)
+ * on a density basis: vion_dev = sum_s S_s(G) Vloc_s(|G|) / omega and rhops_dev =
+ * sum_s S_s(G) (-zval_s exp(-rcps_s^2 G^2 / 4)) / omega, S_s(G) = sum_a exp(-i G.tau_a)
+ * over the na[s] atoms of species s (tau_dev: device, 3 per atom, species after species);
+ * complex per G (overwritten), the inputs of qbh_update_vhxc. */
+int qbh_local_tables(qbh_basis vbasis, int nsp, const qbh_species* sps, const int* na,
+                     const double* tau_dev, double omega, double* vion_dev, double* rhops_dev);
+/* The pseudo-charge repulsion energy esr of EnergyFunctional::atoms_moved
+ * (EnergyFunctional.cc:1918-1988): pairs of distinct atoms, each separation folded into
+ * the Wigner-Seitz cell and summed over (2 n + 1)^3 images, n = int(2.5 rckj / (shortest
+ * of the 13 Wigner-Seitz translations) + 0.5), rckj^2 = rcps_k^2 + rcps_j^2.  a: 3 x 3
+ * host, rows = cell vectors; tau: host, 3 per atom, species after species. */
+int qbh_ewald_esr(const double* a, int nsp, const qbh_species* sps, const int* na,
+                  const double* tau, double* esr);
 
 /* One exponential of the ETRS propagator (ExponentialWavefunctionStepper::exponential,
  * ExponentialWavefunctionStepper.cc:51-148): psi <- sum_{N<=order} (-i dt)^N/N! H^N psi

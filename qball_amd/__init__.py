@@ -6,9 +6,9 @@ behind qball-shaped classes.  This Python package is a thin ctypes binding of th
 ABI, used by the tests and the benchmark; it refuses to run without the built library.
 """
 from .capi import (QBH_ALL, QBH_ACCUMULATE, QBH_FP32, QBH_KINETIC, QBH_LOCAL, QBH_NONLOCAL,
-                   Basis, Batch, Context, Hamiltonian, QbhError, TDPropagator, factorizable, lib,
-                   rods_build)
+                   Basis, Batch, Context, Hamiltonian, QbhError, Species, TDPropagator,
+                   ewald_esr, factorizable, lib, local_tables, rods_build)
 
 __all__ = ["QBH_ALL", "QBH_ACCUMULATE", "QBH_FP32", "QBH_KINETIC", "QBH_LOCAL",
-           "QBH_NONLOCAL", "Basis", "Batch", "Context", "Hamiltonian", "QbhError", "TDPropagator",
-           "factorizable", "lib", "rods_build"]
+           "QBH_NONLOCAL", "Basis", "Batch", "Context", "Hamiltonian", "QbhError", "Species",
+           "TDPropagator", "ewald_esr", "factorizable", "lib", "local_tables", "rods_build"]

@@ -86,6 +86,16 @@ SIGNATURES = {
                                   ctypes.POINTER(_c_long), _ip, ctypes.POINTER(_vp),
                                   ctypes.POINTER(_c_long), _c_int, _c_int, ctypes.POINTER(_vp)]),
     "qbh_batch_launch": (_c_int, [_vp]),
+    "qbh_species_create": (_c_int, [_c_double, _c_int, _c_int, _c_double, _c_int, _dp, _dp,
+                                    _c_double, ctypes.POINTER(_vp)]),
+    "qbh_species_create_xml": (_c_int, [ctypes.c_char_p, _c_double, ctypes.POINTER(_vp)]),
+    "qbh_species_destroy": (_c_int, [_vp]),
+    "qbh_species_info": (_c_int, [_vp, _dp, _dp, _dp, _ip, _ip, _dp]),
+    "qbh_species_eval": (_c_int, [_vp, _c_int, _c_int, _dp, _dp, _dp]),
+    "qbh_species_twnl": (_c_int, [_vp, _vp, _dp, _vp]),
+    "qbh_local_tables": (_c_int, [_vp, _c_int, ctypes.POINTER(_vp), _ip, _vp, _c_double, _vp,
+                                  _vp]),
+    "qbh_ewald_esr": (_c_int, [_dp, _c_int, ctypes.POINTER(_vp), _ip, _dp, _dp]),
     "qbh_batch_destroy": (_c_int, [_vp]),
     "qbh_td_create": (_c_int, [_vp, _vp, _c_int, _dp, _c_double, _c_double, _vp, _vp,
                                ctypes.POINTER(_vp)]),
@@ -285,6 +295,83 @@ class Basis:
         if self.h:
             lib().qbh_basis_destroy(self.h)
             self.h = _vp()
+
+
+class Species:
+    """Radial tables of a norm-conserving KB species (qbh_species_*), from a species XML
+    file (path) or from radial arrays (zval, lmax, llocal, deltar, vps, phi); rcps is the
+    Gaussian width of the ion pseudo-charge."""
+
+    def __init__(self, path=None, rcps=1.5, zval=None, lmax=None, llocal=None, deltar=None,
+                 vps=None, phi=None):
+        self.h = _vp()
+        if path is not None:
+            rc = lib().qbh_species_create_xml(str(path).encode(), float(rcps), ctypes.byref(self.h))
+        else:
+            v = np.ascontiguousarray(vps, dtype=np.float64)
+            f = np.ascontiguousarray(phi, dtype=np.float64)
+            npts = v.shape[1]
+            rc = lib().qbh_species_create(float(zval), int(lmax), int(llocal), float(deltar), npts,
+                                          v.ctypes.data_as(_dp), f.ctypes.data_as(_dp),
+                                          float(rcps), ctypes.byref(self.h))
+        if rc != 0:
+            raise QbhError(f"qbh_species_create failed ({rc})")
+        z, rc_, es = ctypes.c_double(), ctypes.c_double(), ctypes.c_double()
+        n = ctypes.c_int()
+        lib().qbh_species_info(self.h, ctypes.byref(z), ctypes.byref(rc_), ctypes.byref(es),
+                               ctypes.byref(n), None, None)
+        self.zval, self.rcps, self.eself, self.nproj = z.value, rc_.value, es.value, n.value
+        self.lproj = np.zeros(max(n.value, 1), dtype=np.int32)
+        self.wt = np.zeros(max(n.value, 1))
+        lib().qbh_species_info(self.h, None, None, None, None, self.lproj.ctypes.data_as(_ip),
+                               self.wt.ctypes.data_as(_dp))
+        self.lproj, self.wt = self.lproj[:n.value], self.wt[:n.value]
+
+    def eval(self, which, g):
+        """(v, dv) of Vloc(G) (which = -1) or of the l = which projector at |G| = g."""
+        g_, gp = _da(g)
+        v = np.zeros(g_.size)
+        dv = np.zeros(g_.size)
+        rc = lib().qbh_species_eval(self.h, int(which), g_.size, gp, v.ctypes.data_as(_dp),
+                                    dv.ctypes.data_as(_dp))
+        if rc != 0:
+            raise QbhError(f"qbh_species_eval failed ({rc})")
+        return v, dv
+
+    def twnl(self, basis, twnl_ptr, A=None):
+        """Projector tables (nproj rows of ngw) at |k+G-A| into device memory twnl_ptr."""
+        a_ = None if A is None else np.ascontiguousarray(A, dtype=np.float64)
+        basis.ctx.check(lib().qbh_species_twnl(self.h, basis.h,
+                                               None if a_ is None else a_.ctypes.data_as(_dp),
+                                               twnl_ptr), "qbh_species_twnl")
+
+    def close(self):
+        if self.h:
+            lib().qbh_species_destroy(self.h)
+            self.h = _vp()
+
+
+def local_tables(vbasis, species, na, tau_ptr, omega, vion_ptr, rhops_ptr):
+    """v_ion(G) and rho_ps(G) on the density basis (qbh_local_tables); tau_ptr: device,
+    3 doubles per atom, species after species."""
+    sp = (_vp * max(len(species), 1))(*[s.h for s in species])
+    na_, nap = _ia(na)
+    vbasis.ctx.check(lib().qbh_local_tables(vbasis.h, len(species), sp, nap, tau_ptr,
+                                            float(omega), vion_ptr, rhops_ptr),
+                     "qbh_local_tables")
+
+
+def ewald_esr(a, species, na, tau):
+    """esr of EnergyFunctional::atoms_moved (qbh_ewald_esr); tau host (natoms, 3)."""
+    a_, ap = _da(np.asarray(a, dtype=np.float64).reshape(9))
+    sp = (_vp * max(len(species), 1))(*[s.h for s in species])
+    na_, nap = _ia(na)
+    t_, tp = _da(np.asarray(tau, dtype=np.float64).reshape(-1))
+    out = ctypes.c_double(0.0)
+    rc = lib().qbh_ewald_esr(ap, len(species), sp, nap, tp, ctypes.byref(out))
+    if rc != 0:
+        raise QbhError(f"qbh_ewald_esr failed ({rc})")
+    return out.value
 
 
 class Hamiltonian:
