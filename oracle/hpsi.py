@@ -11,6 +11,16 @@ from . import fft as offt
 
 # --------------------------------------------------------------------------------------
 # local potential
+def _re(x):
+    """Contiguous copy of the real part: numpy's matmul only hands contiguous operands to
+    BLAS (a strided .real view takes a plain loop, hundreds of times slower)."""
+    return np.ascontiguousarray(x.real)
+
+
+def _im(x):
+    return np.ascontiguousarray(x.imag)
+
+
 def rs_mul_add(maps, c, v, cp):
     """SlaterDet::rs_mul_add (SlaterDet.cc:971-1040): cp[:, n] += FFT^-1[v * FFT c_n].
 
@@ -38,6 +48,31 @@ def rs_mul_add(maps, c, v, cp):
         for n in range(nst):
             f = offt.backward(maps, c[:, n])
             cp[:, n] += offt.forward(maps, vg * f, pair=False)
+    return cp
+
+
+def rs_mul_add_batched(maps, c, v, cp, batch=8, plan=None):
+    """rs_mul_add through the pruned pencil transforms of oracle.fft.PrunedPlan, `batch`
+    transforms at a time.  Γ: bands in pairs; an odd last band alone with the imaginary
+    part of v*psi zeroed (SlaterDet.cc:1007-1023).  Complex basis: one band per transform."""
+    plan = plan or offt.PrunedPlan(maps)
+    vt = np.ascontiguousarray(v.reshape(maps.np2, maps.np1, maps.np0).transpose(2, 1, 0))
+    vt = vt[:, :, :, None]  # (np0, np1, np2, 1): the plan's grid layout
+    nst = c.shape[1]
+    real = maps.basis.real
+    step = 2 * batch if real else batch
+    npair = 2 * (nst // 2) if real else nst
+    for n0 in range(0, npair, step):
+        n1 = min(npair, n0 + step)
+        f = plan.bwd(c[:, n0:n1], pair=real)
+        f.real *= vt
+        f.imag *= vt
+        cp[:, n0:n1] += plan.fwd(f, n1 - n0, pair=real)
+    if real and nst % 2:
+        n = nst - 1
+        f = plan.bwd(c[:, n:n + 1], pair=False)
+        f = (vt * f.real).astype(np.complex128)
+        cp[:, n:n + 1] += plan.fwd(f, 1, pair=False)
     return cp
 
 
@@ -94,20 +129,25 @@ def anl_block(sp, kpgx, ia0, ia1):
     sn = np.sin(kpgr)
     ngw = kpgx.shape[0]
     anl = np.empty((ngw, sp.npr * nblk), dtype=np.complex128)
+    re, im = anl.real, anl.imag   # views: each block is written in place
     for ipr in range(sp.npr):
         t = sp.twnl[ipr][:, None]
         l = int(sp.lproj[ipr])
-        if l == 0:
-            a = t * cs + 1j * (t * sn)
-        elif l == 1:
-            a = t * sn + 1j * (-t * cs)
-        elif l == 2:
-            a = -t * cs + 1j * (-t * sn)
-        elif l == 3:
-            a = -t * sn + 1j * (t * cs)
+        cols = slice(ipr * nblk, (ipr + 1) * nblk)
+        if l == 0:      # twnl * (cos, sin)
+            np.multiply(t, cs, out=re[:, cols])
+            np.multiply(t, sn, out=im[:, cols])
+        elif l == 1:    # twnl * (sin, -cos)
+            np.multiply(t, sn, out=re[:, cols])
+            np.multiply(-t, cs, out=im[:, cols])
+        elif l == 2:    # twnl * (-cos, -sin)
+            np.multiply(-t, cs, out=re[:, cols])
+            np.multiply(-t, sn, out=im[:, cols])
+        elif l == 3:    # twnl * (-sin, cos)
+            np.multiply(-t, sn, out=re[:, cols])
+            np.multiply(t, cs, out=im[:, cols])
         else:
             raise ValueError("l > 3")
-        anl[:, ipr * nblk:(ipr + 1) * nblk] = a
     return anl
 
 
@@ -125,6 +165,8 @@ def nonlocal_energy(basis, species, c, occ, omega, cp=None, kpgx=None, na_block=
     if na_block is None:
         na_block = 128 if namax > 128 else namax     # :1535-1541
     enl = 0.0
+    if basis.real:
+        cr, ci = _re(c), _im(c)
     for sp in species:
         if sp.npr == 0:
             continue
@@ -135,7 +177,8 @@ def nonlocal_energy(basis, species, c, occ, omega, cp=None, kpgx=None, na_block=
             nblk = ia1 - ia0
             anl = anl_block(sp, kpgx, ia0, ia1)
             if basis.real:
-                fnl = anl.real.T @ c.real + anl.imag.T @ c.imag
+                ar, ai = _re(anl), _im(anl)
+                fnl = ar.T @ cr + ai.T @ ci
                 fnl = fnl - 0.5 * np.outer(anl[0].real, c[0].real)
                 fnl = 2.0 * fnl
             else:
@@ -151,7 +194,8 @@ def nonlocal_energy(basis, species, c, occ, omega, cp=None, kpgx=None, na_block=
                 fnl[rows] = fac * t
             if cp is not None:
                 if basis.real:
-                    cp += anl.real @ fnl + 1j * (anl.imag @ fnl)
+                    cp.real += ar @ fnl
+                    cp.imag += ai @ fnl
                 else:
                     cp += anl @ fnl
     return enl
@@ -217,7 +261,7 @@ def nonlocal_forces(basis, species, c, occ, omega, kpgx=None, na_block=None):
             nblk = ia1 - ia0
             anl = anl_block(sp, kpgx, ia0, ia1)
             if basis.real:
-                fnl = anl.real.T @ c.real + anl.imag.T @ c.imag
+                fnl = _re(anl).T @ _re(c) + _im(anl).T @ _im(c)
                 fnl = fnl - 0.5 * np.outer(anl[0].real, c[0].real)
                 fnl = 2.0 * fnl
             else:
@@ -234,7 +278,7 @@ def nonlocal_forces(basis, species, c, occ, omega, kpgx=None, na_block=None):
             for j in range(3):
                 danl = danl_block(sp, kpgx, j, ia0, ia1)
                 if basis.real:
-                    dfnl = 2.0 * (danl.real.T @ c.real + danl.imag.T @ c.imag)
+                    dfnl = 2.0 * (_re(danl).T @ _re(c) + _im(danl).T @ _im(c))
                     prod = fnl * dfnl
                 else:
                     dfnl = danl.conj().T @ c
@@ -291,7 +335,7 @@ def gram(c, real):
     Complex: S = C^H C, C <- C L^-H."""
     if real:
         cr = np.concatenate([c.real, c.imag], axis=0)
-        s = 2.0 * (c.real.T @ c.real + c.imag.T @ c.imag) - np.outer(c[0].real, c[0].real)
+        s = 2.0 * (_re(c).T @ _re(c) + _im(c).T @ _im(c)) - np.outer(c[0].real, c[0].real)
         L = np.linalg.cholesky(s)
         x = np.linalg.solve(L, cr.T).T        # X L^T = C  <=>  L X^T = C^T
         ngw = c.shape[0]
@@ -307,7 +351,7 @@ def subspace_diag(c, hc, real):
     row 0, :1546-1548), lower triangle to syevd; complex: H = C^H (HC), heevd.  Returns
     (eigenvalues ascending, C Z)."""
     if real:
-        h = 2.0 * (c.real.T @ hc.real + c.imag.T @ hc.imag) - np.outer(c[0].real, hc[0].real)
+        h = 2.0 * (_re(c).T @ _re(hc) + _im(c).T @ _im(hc)) - np.outer(c[0].real, hc[0].real)
         w, z = np.linalg.eigh(h, UPLO="L")
     else:
         h = c.conj().T @ hc

@@ -31,7 +31,8 @@ def prec_diag(basis, ecutprec):
 def _metric_product(c, hc, real):
     """A = 2 C^T HC - c0 hc0^T over the real view (Γ) or C^H HC."""
     if real:
-        a = 2.0 * (c.real.T @ hc.real + c.imag.T @ hc.imag)
+        a = 2.0 * (np.ascontiguousarray(c.real).T @ np.ascontiguousarray(hc.real)
+                   + np.ascontiguousarray(c.imag).T @ np.ascontiguousarray(hc.imag))
         a -= np.outer(c[0].real, hc[0].real)
         return a
     return c.conj().T @ hc

@@ -117,3 +117,22 @@ def test_parseval():
     c = rng.standard_normal(b.size) + 1j * rng.standard_normal(b.size)
     f = offt.backward(maps, c)
     assert abs(np.sum(np.abs(f) ** 2) / f.size - np.sum(np.abs(c) ** 2)) <= 1e-12 * np.sum(np.abs(c) ** 2)
+
+
+@pytest.mark.parametrize("real,nb", [(True, 7), (True, 10), (False, 5)])
+def test_pruned_batched_local_term_matches_full_transforms(real, nb):
+    """rs_mul_add through the pruned pencil transforms (PrunedPlan, several transforms per
+    batch, odd band count on Γ) equals rs_mul_add through the full 3-D transforms."""
+    from helpers import oracle_for
+    from oracle import hpsi as oh
+    from qball_amd.workload import make_system
+    s = make_system(ncell=1, ecut_ry=25.0, nbands=nb, seed=11, complex_basis=not real)
+    _, maps, _ = oracle_for(s)
+    ref = oh.rs_mul_add(maps, s.psi, s.v, np.zeros_like(s.psi))
+    got = oh.rs_mul_add_batched(maps, s.psi, s.v, np.zeros_like(s.psi), batch=2)
+    assert np.max(np.abs(got - ref)) <= 1e-13 * np.max(np.abs(ref))
+    c = s.psi[:, :4]
+    plan = offt.PrunedPlan(maps)
+    f = plan.backward(c, pair=False)
+    for j in range(4):
+        assert np.max(np.abs(f[j] - offt.backward(maps, c[:, j]))) <= 1e-12
