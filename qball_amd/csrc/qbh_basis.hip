@@ -36,8 +36,11 @@ TimedLaunch::~TimedLaunch() {
   }
 }
 
+thread_local bool t_capturing = false;
+
 void* ws_get(Ctx* ctx, size_t bytes) {
   if (bytes <= ctx->ws_bytes) return ctx->ws;
+  if (t_capturing) return nullptr;  // no free / malloc / synchronisation under capture
   if (ctx->ws) {
     hipStreamSynchronize(ctx->stream);
     hipFree(ctx->ws);

@@ -129,6 +129,12 @@ struct TimedLaunch {
 
 void* ws_get(Ctx* ctx, size_t bytes);
 
+// Set (per host thread) while the multi-cell Hψ graph is being captured: every workspace,
+// table and handle must then already exist, so the growth paths (ws_get, grow), the
+// occupation upload and the Enl read-back refuse to allocate or synchronise and fail with
+// an error instead.
+extern thread_local bool t_capturing;
+
 // FFT pipeline entry points (qbh_fft.hip)
 enum SlotMode { SLOT_PAIR = 0, SLOT_SINGLE_REAL = 1, SLOT_COMPLEX = 2 };
 enum Prec { PREC_FP64 = 0, PREC_FP32 = 1 };

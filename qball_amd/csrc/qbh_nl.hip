@@ -31,6 +31,7 @@ namespace qbh {
 
 void* grow(void*& p, size_t& have, size_t need) {
   if (need <= have && p) return p;
+  if (t_capturing) return nullptr;  // no free / malloc under graph capture
   if (p) hipFree(p);
   p = nullptr;
   have = 0;
@@ -366,6 +367,10 @@ static int ensure_anl(Ham* h, bool fp32) {
   Ctx* c = h->ctx;
   const int ngw = h->b->g.ngw;
   const size_t n = 2 * (size_t)ngw * h->P;
+  if (t_capturing && ((!h->anl64 && !h->anl64_ok) || (fp32 && !h->anl32 && !h->anl32_ok))) {
+    c->last_error = "projector table would be allocated during graph capture";
+    return QBH_ERR_ALLOC;
+  }
   if (!h->anl64_ok) {
     if (!h->anl64 && hipMalloc(&h->anl64, n * sizeof(double)) != hipSuccess) {
       h->anl64 = nullptr;
@@ -490,6 +495,10 @@ constexpr int kRedBlocks = 512;
 static int nl_occ(Ham* h, int nst, const double* occ_host, const double** occ) {
   Ctx* c = h->ctx;
   if (h->red_n < (size_t)(kRedBlocks + 1 + nst)) {
+    if (t_capturing) {
+      c->last_error = "reduction scratch would be allocated during graph capture";
+      return QBH_ERR_ALLOC;
+    }
     if (h->red) hipFree(h->red);
     h->red = nullptr;
     h->red_n = 0;
@@ -506,6 +515,10 @@ static int nl_occ(Ham* h, int nst, const double* occ_host, const double** occ) {
   if (occ_host)
     for (int n = 0; n < nst; ++n) o[n] = occ_host[n];
   if (o != h->occ_host) {
+    if (t_capturing) {
+      c->last_error = "occupations would be uploaded (a synchronising copy) during graph capture";
+      return QBH_ERR_UNSUPPORTED;
+    }
     hipMemcpyAsync(d, o.data(), nst * sizeof(double), hipMemcpyHostToDevice, c->stream);
     hipStreamSynchronize(c->stream);  // o is a stack buffer
     h->occ_host = o;

@@ -372,12 +372,14 @@ extern "C" int qbh_batch_create(qbh_ctx ctx, int ncells, const qbh_ham* hams,
                                      "non-default stream)");
   }
   hipEventRecord(bt->events[0], s0);
+  t_capturing = true;  // growth, uploads and read-backs fail instead of running
   for (int b = 0; b < nbranch && r == QBH_OK; ++b) {
     hipStreamWaitEvent(bt->branch[b]->stream, bt->events[0], 0);
     r = run_branch(bt, b);
     hipEventRecord(bt->events[1 + b], bt->branch[b]->stream);
     hipStreamWaitEvent(s0, bt->events[1 + b], 0);
   }
+  t_capturing = false;
   hipGraph_t g = nullptr;
   const hipError_t ec = hipStreamEndCapture(s0, &g);
   bt->graph = g;

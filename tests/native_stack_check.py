@@ -7,6 +7,13 @@ projector GEMM choice (the library's matrix-core kernels), rocBLAS, and every al
 hipBLASLt lists, in FP64 and in the FP32
 form, into an output buffer full of NaN (which Hψ without accumulation must not read).
 Prints one JSON line with the errors.
+
+With --fullsize: Hψ of the benchmark's two full-size cells (Si512 Γ, 1024 bands; fcc Al500
+Γ, 900 bands) with all bands at once, so the projector products run at the shapes the
+benchmark times, for the built-in launch choices, the autotuner's choice, rocBLAS and
+hipBLASLt's second listed algorithm (FP64) and the built-in and autotuned FP32 form; the
+first and last band pairs against the oracle (1e-12; FP32 2e-6) and the FP32 form against
+FP64 over all bands.
 """
 import json
 import os
@@ -75,5 +82,72 @@ def main():
     assert "torch" not in sys.modules
 
 
+def fullsize():
+    from helpers import oracle_for, rel_err
+    from oracle import hpsi as oh
+    from qball_amd import Basis, Context, Hamiltonian
+    from qball_amd.devmem import DeviceArray
+    from qball_amd.workload import make_fcc_al, make_system
+
+    ctx = Context(0, None)
+    out = {"libraries": ctx.library_info(), "cases": []}
+    knobs = ("zy_reg", "x_reg", "x_spw", "radix", "nsplit", "overlap", "fnl_algo", "enl_algo")
+    for which in ("si512", "al500"):
+        s = make_system(ncell=4, ecut_ry=25.0, seed=0) if which == "si512" else make_fcc_al(seed=1)
+        print(f"native stack check: {which} ngw {s.ngw} nst {s.nst}", file=sys.stderr, flush=True)
+        r = s.rods
+        basis = Basis(ctx, True, r["rod_h"], r["rod_k"], r["rod_lmin"], r["rod_size"], s.np, s.b)
+        ham = Hamiltonian(ctx, basis)
+        vd = DeviceArray.from_numpy(np.ascontiguousarray(s.v))
+        ham.set_vloc(vd.ptr)
+        ham.set_projectors([s.na], [len(s.lproj)], s.lproj, s.wt, s.twnl.ravel(), s.tau.ravel(),
+                           s.omega)
+        ngw, nst = s.ngw, s.nst
+        sub = [0, 1, nst - 2, nst - 1]
+        _, maps, sp = oracle_for(s)
+        ref = oh.hpsi(maps, s.psi[:, sub], s.v, [sp], s.occ[sub], s.omega)[0]
+        psi = DeviceArray.from_numpy(np.ascontiguousarray(s.psi.T).view(np.float64).ravel())
+        hp = DeviceArray(psi.nbytes)
+        h64 = None
+        for flags in (0x7, 0x17):
+            settings = [("builtin", {"zy_reg": 1, "x_reg": 1, "x_spw": 0, "radix": 0,
+                                     "nsplit": 0, "overlap": 0, "fnl_algo": -1,
+                                     "enl_algo": -1})]
+            if flags == 0x7:
+                settings += [("rocblas", {"fnl_algo": 0, "enl_algo": 0}),
+                             ("hipblaslt-2", {"fnl_algo": 2, "enl_algo": 2})]
+            settings.append(("autotuned", None))
+            for name, choice in settings:
+                if choice is None:
+                    ham.autotune(psi.ptr, ngw, nst, s.occ, hp.ptr, ngw, flags=flags, reps=2,
+                                 force=True)
+                else:
+                    for k, v in choice.items():
+                        ham.set_tuning(k, v)
+                hp.upload(np.full(psi.nbytes // 8, np.nan))
+                ham.hpsi(psi.ptr, ngw, nst, s.occ, hp.ptr, ngw, flags=flags)
+                ctx.sync()
+                got = hp.to_numpy().view(np.complex128).reshape(nst, ngw).T
+                case = {"cell": which, "flags": flags, "choice": name,
+                        "knobs": {k: ham.get_tuning(k) for k in knobs},
+                        "err": rel_err(got[:, sub], ref)}
+                if flags == 0x7 and name == "builtin":
+                    h64 = got.copy()
+                elif flags == 0x17:
+                    case["err_vs_fp64"] = rel_err(got, h64)
+                out["cases"].append(case)
+                print(f"native stack check: {json.dumps(case)}", file=sys.stderr, flush=True)
+        ham.close()
+        basis.close()
+        for d in (psi, hp, vd):
+            d.close()
+    ctx.close()
+    print(json.dumps(out))
+    assert "torch" not in sys.modules
+
+
 if __name__ == "__main__":
-    main()
+    if "--fullsize" in sys.argv[1:]:
+        fullsize()
+    else:
+        main()
