@@ -212,6 +212,20 @@ int qbh_species_eval(qbh_species sp, int which, int n, const double* g, double* 
  * d: y4..y8; f: y9..y15).  The rows, qbh_species_info's lproj and wt, and the atom
  * positions are the arguments of qbh_ham_set_projectors. */
 int qbh_species_twnl(qbh_species sp, qbh_basis basis, const double* A, double* twnl_dev);
+/* The gradient of qbh_species_twnl's rows with respect to q = k+G-A: dtwnl_dev (device, 3
+ * blocks of nproj rows of ngw, d = x, y, z) = d twnl / dq_d; zero where q = 0.  The
+ * derivative with respect to A is its negative (the stress tables of update_twnl,
+ * NonLocalPotential.cc:261-1521, differentiate the same rows with respect to strain). */
+int qbh_species_dtwnl(qbh_species sp, qbh_basis basis, const double* A, double* dtwnl_dev);
+/* NonLocalPotential::update_twnl + EnergyFunctional::vector_potential_changed
+ * (EnergyFunctional.cc:2047-2057) behind qbh_ham_set_projectors: the projector tables of
+ * the Hamiltonian from species (rows, lproj, wt as qbh_species_twnl / qbh_species_info give
+ * them; na atoms of each species, tau host 3 per atom species after species), tabled at
+ * |k+G-A| of the Hamiltonian's vector potential and re-tabled by every later
+ * qbh_ham_set_vector_potential (so by the TD propagator at each step).  The species must
+ * outlive the Hamiltonian or its next qbh_ham_set_projectors / qbh_ham_set_species. */
+int qbh_ham_set_species(qbh_ham ham, int nsp, const qbh_species* sps, const int* na,
+                        const double* tau, double omega);
 /* EnergyFunctional::cell_moved + atoms_moved (EnergyFunctional.cc:1889-1915, 2001-2021)
  * on a density basis: vion_dev = sum_s S_s(G) Vloc_s(|G|) / omega and rhops_dev =
  * sum_s S_s(G) (-zval_s exp(-rcps_s^2 G^2 / 4)) / omega, S_s(G) = sum_a exp(-i G.tau_a)

@@ -341,6 +341,64 @@ def twnl_table(sp, kpgx):
     return np.array(rows), np.array(ls), np.array(wts)
 
 
+def _harmonic_polys(l, ux, uy, uz):
+    """The angular factors of twnl_table's rows of angular momentum l as polynomials Y(u)
+    of the direction components, with their partial derivatives (dY/dux, dY/duy, dY/duz)
+    taken with the components independent."""
+    fpi = 4.0 * math.pi
+    z = np.zeros_like(ux)
+    if l == 0:
+        c = math.sqrt(1.0 / fpi)
+        return [(c + z, z, z, z)]
+    if l == 1:
+        c = math.sqrt(3.0 / fpi)
+        o = c + z
+        return [(c * ux, o, z, z), (c * uy, z, o, z), (c * uz, z, z, o)]
+    if l == 2:
+        c = math.sqrt(5.0 / fpi)
+        s3 = math.sqrt(3.0)
+        return [(c * 0.5 * (3.0 * uz * uz - 1.0), z, z, c * 3.0 * uz),
+                (c * 0.5 * s3 * (ux * ux - uy * uy), c * s3 * ux, -c * s3 * uy, z),
+                (c * s3 * ux * uy, c * s3 * uy, c * s3 * ux, z),
+                (c * s3 * uy * uz, z, c * s3 * uz, c * s3 * uy),
+                (c * s3 * uz * ux, c * s3 * uz, z, c * s3 * ux)]
+    a = math.sqrt(7.0 / fpi)
+    b = math.sqrt(21.0 / (32.0 * math.pi))
+    c = math.sqrt(35.0 / (32.0 * math.pi))
+    d = math.sqrt(105.0 / fpi)
+    return [(a * 0.5 * uz * (5.0 * uz * uz - 3.0), z, z, a * 0.5 * (15.0 * uz * uz - 3.0)),
+            (b * ux * (5.0 * uz * uz - 1.0), b * (5.0 * uz * uz - 1.0), z, b * 10.0 * ux * uz),
+            (b * uy * (5.0 * uz * uz - 1.0), z, b * (5.0 * uz * uz - 1.0), b * 10.0 * uy * uz),
+            (d * ux * uy * uz, d * uy * uz, d * ux * uz, d * ux * uy),
+            (d * 0.5 * uz * (ux * ux - uy * uy), d * uz * ux, -d * uz * uy,
+             d * 0.5 * (ux * ux - uy * uy)),
+            (c * ux * (ux * ux - 3.0 * uy * uy), c * 3.0 * (ux * ux - uy * uy), -c * 6.0 * ux * uy, z),
+            (c * uy * (3.0 * ux * ux - uy * uy), c * 6.0 * ux * uy, c * 3.0 * (ux * ux - uy * uy), z)]
+
+
+def twnl_grad_table(sp, kpgx):
+    """Gradient of twnl_table's rows with respect to the vector q = k+G(-A): (3, rows, ngw).
+    Each row is v_l(|q|) Y(q/|q|), so d/dq_d = v_l' u_d Y + (v_l/|q|)(dY/du_d - u_d u.grad Y)
+    with u = q/|q|; zero where q = 0 (twnl_table's direction factors vanish there).  The
+    derivative a vector potential A brings in is -d/dq (q = k+G-A).  Pinned by central
+    differences of twnl_table."""
+    kx, ky, kz = kpgx[:, 0], kpgx[:, 1], kpgx[:, 2]
+    k = np.sqrt(kx * kx + ky * ky + kz * kz)
+    ki = np.where(k > 0.0, 1.0 / np.where(k > 0.0, k, 1.0), 0.0)
+    ux, uy, uz = kx * ki, ky * ki, kz * ki
+    u = (ux, uy, uz)
+    out = []
+    for l in range(sp.lmax + 1):
+        if l == sp.llocal:
+            continue
+        v, dv = sp.dvnlg(l, k)
+        for Y, gx, gy, gz in _harmonic_polys(l, ux, uy, uz):
+            g = (gx, gy, gz)
+            ug = ux * gx + uy * gy + uz * gz
+            out.append([dv * u[d] * Y + v * ki * (g[d] - u[d] * ug) for d in range(3)])
+    return np.array(out).transpose(1, 0, 2)
+
+
 def structure_factor(tau, gx):
     """StructureFactor::update (StructureFactor.cc:85-173): sum_a exp(-i G.tau_a)."""
     arg = gx @ np.asarray(tau, dtype=np.float64).T

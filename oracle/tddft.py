@@ -12,9 +12,12 @@ for fixed ions, one spin, one k-point, complex basis:
 * update (ExponentialWavefunctionStepper.cc:243-276): v_b = update_vhxc(rho(psi~)) + v_ext
   and psi_{n+1} = T(dt/2) psi(t+dt/2) under H[v_b, A_{n+1}].
 
-T is the order-4 Taylor exponential (ExponentialWavefunctionStepper.cc:51-148).  The KB
-radial tables stay at |k+G| (no re-tabling at |k+G-A|); the projector phases at k+G-A
-cancel exactly in beta D beta^H, so A enters H only through 0.5|k+G-A|^2.
+T is the order-4 Taylor exponential (ExponentialWavefunctionStepper.cc:51-148).  A enters H
+through 0.5|k+G-A|^2 and, when the system holds the species (``retable``), through the
+projector tables re-tabled at |k+G-A| on every change of A (NonLocalPotential::update_twnl
+under a vector potential, EnergyFunctional::vector_potential_changed,
+EnergyFunctional.cc:2047-2057); given fixed tables (``species``) they stay at |k+G|.  The
+projector phases at k+G-A cancel exactly in beta D beta^H either way.
 
 Objective: J = sum_n w_n e.j(psi_n, A_n) with j_d = -sum_b f_b sum_G (k+G)_d |c_bG|^2 +
 A_d N_e (the G-space form of the total current, equal to the grid sum of update_current
@@ -30,6 +33,7 @@ import numpy as np
 
 from . import fft as offt
 from . import hpsi as oh
+from . import species as osp
 from .vhxc import update_vhxc
 
 
@@ -78,9 +82,13 @@ def hartree_apply(vmaps, mu):
 class TDSystem:
     """Everything the propagation needs besides psi, v_ext and A(t)."""
 
-    def __init__(self, maps, vmaps, species, occ, omega, vion_g, rhops_g, weight=1.0):
+    def __init__(self, maps, vmaps, species, occ, omega, vion_g, rhops_g, weight=1.0,
+                 retable=None):
+        """species: oracle.hpsi.Projectors with fixed tables; or retable: [(oracle.species
+        .Species, tau)] whose tables follow A."""
         self.maps, self.vmaps = maps, vmaps
         self.species = species
+        self.retable = retable
         self.occ = np.asarray(occ, dtype=np.float64)
         self.omega = omega
         self.vion_g, self.rhops_g = vion_g, rhops_g
@@ -95,13 +103,47 @@ class TDSystem:
             v = v + vext
         return v, rho
 
+    def projectors(self, A):
+        """Projectors at A: the fixed ones, or the species' tables at |k+G-A|."""
+        if self.retable is None:
+            return self.species
+        kpgx = self.basis.kpgx - np.asarray(A, dtype=np.float64)[None, :]
+        out = []
+        for sp, tau in self.retable:
+            rows, ls, wt = osp.twnl_table(sp, kpgx)
+            out.append(oh.Projectors(ls, wt, rows, tau))
+        return out
+
     def apply_h(self, v, A):
         b = self.basis
         kpg2 = oh.kpgpa2(b, A)
         kpgx = b.kpgx - np.asarray(A, dtype=np.float64)[None, :]
         occ0 = np.zeros(len(self.occ))
-        return lambda w: oh.hpsi(self.maps, w, v, self.species, occ0, self.omega, kpg2=kpg2,
+        proj = self.projectors(A)
+        return lambda w: oh.hpsi(self.maps, w, v, proj, occ0, self.omega, kpg2=kpg2,
                                  kpgx=kpgx)[0]
+
+    def nl_dA(self, A, z, U):
+        """2 Re sum_b <z_b| dH_NL/dA_d |U_b> (3,) for tables re-tabled at |k+G-A|:
+        dH_NL/dA_d = sum_p |d_d beta_p> D_p <beta_p| + h.c. with d_d beta_p = -(-i)^l
+        (d twnl_p / dq_d) exp(-i (k+G-A).tau) (the A-derivative of the phases cancels)."""
+        out = np.zeros(3)
+        if self.retable is None:
+            return out
+        kpgx = self.basis.kpgx - np.asarray(A, dtype=np.float64)[None, :]
+        for sp, tau in self.retable:
+            rows, ls, wt = osp.twnl_table(sp, kpgx)
+            drows = osp.twnl_grad_table(sp, kpgx)
+            na = len(tau)
+            fac = np.repeat(np.asarray(wt) / self.omega, na)   # column ia + ipr * na
+            anl = oh.anl_block(oh.Projectors(ls, wt, rows, tau), kpgx, 0, na)
+            Fz, FU = anl.conj().T @ z, anl.conj().T @ U
+            for d in range(3):
+                dbeta = -oh.anl_block(oh.Projectors(ls, wt, drows[d], tau), kpgx, 0, na)
+                dFz, dFU = dbeta.conj().T @ z, dbeta.conj().T @ U
+                out[d] += 2.0 * float(np.sum(fac[:, None] * (np.conj(dFz) * FU +
+                                                             np.conj(Fz) * dFU)).real)
+        return out
 
     def current(self, c, A):
         """(total current (3,), N_e): G-space form of CurrentDensity::update_current."""
@@ -166,9 +208,11 @@ def _expo_adjoint(apply_h, lam, dt, order):
     return expo_krylov(apply_h, lam, -dt, order)
 
 
-def _cross(sysm, z, u, dt, order, kvec):
-    """(mu(r), gA) of one exponential: mu = (2/N) Re sum_b sum_k conj(z_k(r)) U_k(r) with
-    U_k = sum_{m <= order-1-k} alpha_{k+m+1} u_m; gA_d = -2 Re sum conj(z_k) kvec_d U_k."""
+def _cross(sysm, z, u, dt, order, A):
+    """(mu(r), gA) of one exponential under A: mu = (2/N) Re sum_b sum_k conj(z_k(r)) U_k(r)
+    with U_k = sum_{m <= order-1-k} alpha_{k+m+1} u_m; gA_d = -2 Re sum conj(z_k) kvec_d U_k
+    (kvec = k+G-A) plus, with re-tabled projectors, 2 Re sum <z_k| dH_NL/dA_d |U_k>."""
+    kvec = sysm.basis.kpgx - np.asarray(A, dtype=np.float64)[None, :]
     a = taylor_coeffs(dt, order)
     maps = sysm.maps
     N = maps.N
@@ -181,6 +225,7 @@ def _cross(sysm, z, u, dt, order, kvec):
             Ur = offt.backward(maps, U[:, b]).ravel()
             mu += (2.0 / N) * (np.conj(zr) * Ur).real
         gA += -2.0 * np.einsum("gb,gd,gb->d", np.conj(z[k]), kvec, U).real
+        gA += sysm.nl_dA(A, z[k], U)
     return mu, gA
 
 
@@ -212,20 +257,18 @@ def td_gradient(sysm, psi0, vext, A, w, e, dt, nsteps, order=4):
     h = 0.5 * dt
     for n in range(nsteps - 1, -1, -1):
         t = traj[n]
-        kb = b.kpgx - A[n + 1][None, :]
-        ka = b.kpgx - A[n][None, :]
         lam_h, z3 = _expo_adjoint(t["Hb"], lam, h, order)
-        mu_b, g = _cross(sysm, z3, t["u3"], h, order, kb)
+        mu_b, g = _cross(sysm, z3, t["u3"], h, order, A[n + 1])
         gA[n + 1] += g
         gv += mu_b
         nu_b = fxc_pz81(t["rho_b"]) * mu_b + hartree_apply(sysm.vmaps, mu_b)
         lam_t = _density_adjoint(sysm, t["tilde"], nu_b)
         l2, z2 = _expo_adjoint(t["Ha"], lam_t, h, order)
         lam_h = lam_h + l2
-        mu_a, g = _cross(sysm, z2, t["u2"], h, order, ka)
+        mu_a, g = _cross(sysm, z2, t["u2"], h, order, A[n])
         gA[n] += g
         lam_psi, z1 = _expo_adjoint(t["Ha"], lam_h, h, order)
-        m1, g = _cross(sysm, z1, t["u1"], h, order, ka)
+        m1, g = _cross(sysm, z1, t["u1"], h, order, A[n])
         mu_a += m1
         gA[n] += g
         gv += mu_a

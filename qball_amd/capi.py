@@ -93,6 +93,8 @@ SIGNATURES = {
     "qbh_species_info": (_c_int, [_vp, _dp, _dp, _dp, _ip, _ip, _dp]),
     "qbh_species_eval": (_c_int, [_vp, _c_int, _c_int, _dp, _dp, _dp]),
     "qbh_species_twnl": (_c_int, [_vp, _vp, _dp, _vp]),
+    "qbh_species_dtwnl": (_c_int, [_vp, _vp, _dp, _vp]),
+    "qbh_ham_set_species": (_c_int, [_vp, _c_int, ctypes.POINTER(_vp), _ip, _dp, _c_double]),
     "qbh_local_tables": (_c_int, [_vp, _c_int, ctypes.POINTER(_vp), _ip, _vp, _c_double, _vp,
                                   _vp]),
     "qbh_ewald_esr": (_c_int, [_dp, _c_int, ctypes.POINTER(_vp), _ip, _dp, _dp]),
@@ -345,6 +347,13 @@ class Species:
                                                None if a_ is None else a_.ctypes.data_as(_dp),
                                                twnl_ptr), "qbh_species_twnl")
 
+    def dtwnl(self, basis, dtwnl_ptr, A=None):
+        """d twnl / d(k+G-A) (3 blocks of nproj rows of ngw) into device memory dtwnl_ptr."""
+        a_ = None if A is None else np.ascontiguousarray(A, dtype=np.float64)
+        basis.ctx.check(lib().qbh_species_dtwnl(self.h, basis.h,
+                                                None if a_ is None else a_.ctypes.data_as(_dp),
+                                                dtwnl_ptr), "qbh_species_dtwnl")
+
     def close(self):
         if self.h:
             lib().qbh_species_destroy(self.h)
@@ -390,6 +399,17 @@ class Hamiltonian:
         self.ctx.check(lib().qbh_ham_set_projectors(self.h, len(keep[0][0]),
                                                     *[k[1] for k in keep], float(omega)),
                        "qbh_ham_set_projectors")
+
+    def set_species(self, species, na, tau, omega):
+        """Projector tables from Species objects (qbh_ham_set_species), re-tabled at |k+G-A|
+        whenever the vector potential changes; the species must stay open meanwhile."""
+        sp = (_vp * max(len(species), 1))(*[s.h for s in species])
+        na_, nap = _ia(na)
+        t_, tp = _da(np.asarray(tau, dtype=np.float64).reshape(-1))
+        self._natoms = int(np.sum(na_))
+        self._species = list(species)
+        self.ctx.check(lib().qbh_ham_set_species(self.h, len(species), sp, nap, tp, float(omega)),
+                       "qbh_ham_set_species")
 
     def set_vector_potential(self, A):
         a_, ap = _da(np.asarray(A, dtype=np.float64))

@@ -304,17 +304,14 @@ extern "C" int qbh_ham_set_vloc(qbh_ham ham, const double* v_dev) {
   return dev_check(h->ctx, "set_vloc");
 }
 
-extern "C" int qbh_ham_set_projectors(qbh_ham ham, int nsp, const int* na, const int* npr,
-                                      const int* lproj, const double* wt, const double* twnl,
-                                      const double* tau, double omega) {
-  if (!ham || nsp < 0 || (nsp > 0 && (!na || !npr || !lproj || !wt || !twnl || !tau)) ||
-      !(omega > 0.0))
-    return QBH_ERR_ARG;
-  Ham* h = &ham->h;
+// projector columns and tables; twnl null: the table is allocated, not filled
+static int set_proj(Ham* h, int nsp, const int* na, const int* npr, const int* lproj,
+                    const double* wt, const double* twnl, const double* tau, double omega) {
   Ctx* c = h->ctx;
   hipSetDevice(c->device);
   hipStreamSynchronize(c->stream);
   free_proj(h);
+  h->sps.clear();
   const int ngw = h->b->g.ngw;
   h->nsp = nsp;
   h->na.assign(na, na + nsp);
@@ -349,7 +346,7 @@ extern "C" int qbh_ham_set_projectors(qbh_ham ham, int nsp, const int* na, const
     using X = std::remove_pointer_t<std::remove_reference_t<decltype(dst)>>;
     void* d = nullptr;
     if (hipMalloc(&d, std::max<size_t>(1, n) * sizeof(X)) != hipSuccess) return false;
-    hipMemcpy(d, src, n * sizeof(X), hipMemcpyHostToDevice);
+    if (src) hipMemcpy(d, src, n * sizeof(X), hipMemcpyHostToDevice);
     dst = static_cast<X*>(d);
     return true;
   };
@@ -363,6 +360,51 @@ extern "C" int qbh_ham_set_projectors(qbh_ham ham, int nsp, const int* na, const
   return dev_check(c, "set_projectors");
 }
 
+extern "C" int qbh_ham_set_projectors(qbh_ham ham, int nsp, const int* na, const int* npr,
+                                      const int* lproj, const double* wt, const double* twnl,
+                                      const double* tau, double omega) {
+  if (!ham || nsp < 0 || (nsp > 0 && (!na || !npr || !lproj || !wt || !twnl || !tau)) ||
+      !(omega > 0.0))
+    return QBH_ERR_ARG;
+  return set_proj(&ham->h, nsp, na, npr, lproj, wt, twnl, tau, omega);
+}
+
+int qbh::retable_twnl(Ham* h) {
+  if (!h->twnl || h->P == 0) return QBH_OK;
+  long row0 = 0;
+  for (qbh_species sp : h->sps) {
+    if (int r = species_table(sp, h->ctx, h->b, h->A, 0, h->twnl + row0 * h->b->g.ngw, 0))
+      return r;
+    row0 += species_nproj(sp);
+  }
+  h->anl64_ok = h->anl32_ok = false;
+  return QBH_OK;
+}
+
+extern "C" int qbh_ham_set_species(qbh_ham ham, int nsp, const qbh_species* sps, const int* na,
+                                   const double* tau, double omega) {
+  if (!ham || nsp < 0 || (nsp > 0 && (!sps || !na || !tau)) || !(omega > 0.0))
+    return QBH_ERR_ARG;
+  Ham* h = &ham->h;
+  std::vector<int> npr(nsp), lproj;
+  std::vector<double> wt;
+  for (int s = 0; s < nsp; ++s) {
+    if (!sps[s]) return QBH_ERR_ARG;
+    int n = 0;
+    qbh_species_info(sps[s], nullptr, nullptr, nullptr, &n, nullptr, nullptr);
+    npr[s] = n;
+    const size_t o = lproj.size();
+    lproj.resize(o + n);
+    wt.resize(o + n);
+    qbh_species_info(sps[s], nullptr, nullptr, nullptr, nullptr, lproj.data() + o, wt.data() + o);
+  }
+  if (int r = set_proj(h, nsp, na, npr.data(), lproj.data(), wt.data(), nullptr, tau, omega))
+    return r;
+  h->sps.assign(sps, sps + nsp);
+  if (int r = retable_twnl(h)) return r;
+  return dev_check(h->ctx, "set_species");
+}
+
 extern "C" int qbh_ham_set_vector_potential(qbh_ham ham, const double* A) {
   if (!ham || !A) return QBH_ERR_ARG;
   Ham* h = &ham->h;
@@ -370,6 +412,9 @@ extern "C" int qbh_ham_set_vector_potential(qbh_ham ham, const double* A) {
   hipStreamSynchronize(h->ctx->stream);
   std::memcpy(h->A, A, 3 * sizeof(double));
   h->anl64_ok = h->anl32_ok = false;
+  // update_twnl at |k+G-A| (EnergyFunctional::vector_potential_changed) when the tables
+  // come from species
+  if (int r = retable_twnl(h)) return r;
   return upload_kin(h);
 }
 

@@ -25,6 +25,9 @@ struct Ham {
   double* col_fac = nullptr;      // per column: wt/Omega
   std::vector<int> col_atom_host; // col_atom on the host (force accumulation)
   double* twnl = nullptr;         // concatenated tables, row = species projector
+  // species the tables were made from (qbh_ham_set_species; empty when the caller passed
+  // the tables): their rows are re-tabled at |k+G-A| whenever A changes
+  std::vector<qbh_species> sps;
   double* tau = nullptr;          // 3 * total atoms
   void* anl64 = nullptr;          // Γ: 2ngw x P real; complex: ngw x P complex
   void* anl32 = nullptr;
@@ -100,6 +103,17 @@ int blas_handle(Ctx* c, void** out);
 // frees a context's rocBLAS handle and hipBLASLt state (qbh_nl.hip)
 void blas_destroy(Ctx* c);
 void* grow(void*& p, size_t& have, size_t need);
+// twnl rows of a species at |k+G-A| (grad 0: rows x ngw into out) or their gradient with
+// respect to k+G-A (grad 1: component d at out + d * dstride), on c's stream (qbh_species.hip)
+int species_table(qbh_species sp, Ctx* c, Basis* b, const double* A, int grad, double* out,
+                  long dstride);
+int species_nproj(qbh_species sp);
+// re-tables the Hamiltonian's twnl rows from its species at its vector potential
+int retable_twnl(Ham* h);
+// complex projector table (ngw x P, ld ngw) of the Hamiltonian's columns built from the rows
+// `rows` (row r of ngw, in the layout of twnl) with phases exp(-i (k+G-A).tau), on the
+// context's stream: anl_p = (-i)^l rows[col_tw[p]] exp(...) (qbh_nl.hip)
+int build_anl_c64(Ham* h, const double* rows, const double* A, double* out);
 
 }  // namespace qbh
 

@@ -363,6 +363,20 @@ static int gemm_fail(Ctx* c, const char* what) {
   return QBH_ERR_ARG;
 }
 
+int build_anl_c64(Ham* h, const double* rows, const double* A, double* out) {
+  const int ngw = h->b->g.ngw;
+  if (h->P == 0) return 0;
+  dim3 grid((ngw + 255) / 256, h->P);
+  k_anl<double><<<grid, 256, 0, h->ctx->stream>>>(ngw, h->col_l, h->col_tw, h->col_atom, rows,
+                                                  h->tau, h->b->kpgx, A[0], A[1], A[2], out);
+  const hipError_t e = hipGetLastError();
+  if (e != hipSuccess) {
+    h->ctx->last_error = std::string("anl: ") + hipGetErrorString(e);
+    return QBH_ERR_DEVICE;
+  }
+  return 0;
+}
+
 static int ensure_anl(Ham* h, bool fp32) {
   Ctx* c = h->ctx;
   const int ngw = h->b->g.ngw;

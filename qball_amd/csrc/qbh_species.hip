@@ -29,6 +29,7 @@
 #include <vector>
 
 #include "../../include/qbh.h"
+#include "qbh_ham.h"
 #include "qbh_internal.h"
 
 namespace qbh {
@@ -405,6 +406,85 @@ k_twnl(int ngw, const double* __restrict__ kpg2, const double* __restrict__ kpgx
   out[6L * ngw + ig] = s3532pi * tgy * tgi * (3.0 * tgxx - tgyy) * v;
 }
 
+// Angular factor m of angular momentum L as a polynomial Y(u) of the direction components
+// (the factors of k_twnl, in its row order) and its partial derivatives with the components
+// taken independent
+template <int L>
+__device__ __forceinline__ void harmonic(int m, double ux, double uy, double uz, double& Y,
+                                         double& gx, double& gy, double& gz) {
+  const double fpi = 4.0 * M_PI;
+  gx = gy = gz = 0.0;
+  if (L == 0) {
+    Y = sqrt(1.0 / fpi);
+  } else if (L == 1) {
+    const double c = sqrt(3.0 / fpi);
+    Y = c * (m == 0 ? ux : m == 1 ? uy : uz);
+    (m == 0 ? gx : m == 1 ? gy : gz) = c;
+  } else if (L == 2) {
+    const double c = sqrt(5.0 / fpi), s3 = sqrt(3.0);
+    switch (m) {
+      case 0: Y = c * 0.5 * (3.0 * uz * uz - 1.0); gz = c * 3.0 * uz; break;
+      case 1: Y = c * 0.5 * s3 * (ux * ux - uy * uy); gx = c * s3 * ux; gy = -c * s3 * uy; break;
+      case 2: Y = c * s3 * ux * uy; gx = c * s3 * uy; gy = c * s3 * ux; break;
+      case 3: Y = c * s3 * uy * uz; gy = c * s3 * uz; gz = c * s3 * uy; break;
+      default: Y = c * s3 * uz * ux; gx = c * s3 * uz; gz = c * s3 * ux; break;
+    }
+  } else {
+    const double a = sqrt(7.0 / fpi), b = sqrt(21.0 / (32. * M_PI)),
+                 c = sqrt(35.0 / (32. * M_PI)), d = sqrt(105.0 / fpi);
+    const double z5 = 5.0 * uz * uz - 1.0;
+    switch (m) {
+      case 0: Y = a * 0.5 * uz * (5.0 * uz * uz - 3.0); gz = a * 0.5 * (15.0 * uz * uz - 3.0); break;
+      case 1: Y = b * ux * z5; gx = b * z5; gz = b * 10.0 * ux * uz; break;
+      case 2: Y = b * uy * z5; gy = b * z5; gz = b * 10.0 * uy * uz; break;
+      case 3: Y = d * ux * uy * uz; gx = d * uy * uz; gy = d * ux * uz; gz = d * ux * uy; break;
+      case 4:
+        Y = d * 0.5 * uz * (ux * ux - uy * uy);
+        gx = d * uz * ux; gy = -d * uz * uy; gz = d * 0.5 * (ux * ux - uy * uy);
+        break;
+      case 5:
+        Y = c * ux * (ux * ux - 3.0 * uy * uy);
+        gx = c * 3.0 * (ux * ux - uy * uy); gy = -c * 6.0 * ux * uy;
+        break;
+      default:
+        Y = c * uy * (3.0 * ux * ux - uy * uy);
+        gx = c * 6.0 * ux * uy; gy = c * 3.0 * (ux * ux - uy * uy);
+        break;
+    }
+  }
+}
+
+// Gradient of the 2L+1 rows of k_twnl with respect to q = k+G-A: out[d * dstride + m * ngw
+// + ig] = d/dq_d [v(|q|) Y_m(q/|q|)] = v' u_d Y + (v / |q|)(dY/du_d - u_d u.grad Y), zero at
+// q = 0 (where k_twnl's direction factors vanish)
+template <int L>
+__global__ void __launch_bounds__(256)
+k_dtwnl(int ngw, const double* __restrict__ kpg2, const double* __restrict__ kpgx, double ax,
+        double ay, double az, const double* __restrict__ gspl, const double* __restrict__ py,
+        const double* __restrict__ py2, int n, double* __restrict__ out, long dstride) {
+  const int ig = blockIdx.x * 256 + threadIdx.x;
+  if (ig >= ngw) return;
+  const double value2 = ax * ax + ay * ay + az * az;
+  double k2 = kpg2[ig] + value2;
+  k2 -= 2 * ax * kpgx[ig];
+  k2 -= 2 * ay * kpgx[ngw + ig];
+  k2 -= 2 * az * kpgx[2 * ngw + ig];
+  const double tg = sqrt(k2);
+  const double tgi = k2 > 0.0 ? 1.0 / sqrt(k2) : 0.0;
+  double v = 0.0, dv = 0.0;
+  if (tg <= gspl[n - 1]) splintd(gspl, py, py2, n, tg, &v, &dv);
+  const double u[3] = {(kpgx[ig] - ax) * tgi, (kpgx[ngw + ig] - ay) * tgi,
+                       (kpgx[2 * ngw + ig] - az) * tgi};
+  for (int m = 0; m < 2 * L + 1; ++m) {
+    double Y, g[3];
+    harmonic<L>(m, u[0], u[1], u[2], Y, g[0], g[1], g[2]);
+    const double ug = u[0] * g[0] + u[1] * g[1] + u[2] * g[2];
+    for (int d = 0; d < 3; ++d)
+      out[d * dstride + (long)m * ngw + ig] =
+          tgi > 0.0 ? dv * u[d] * Y + v * tgi * (g[d] - u[d] * ug) : 0.0;
+  }
+}
+
 // v_ion(G) += S(G) vlocg(|G|) / omega, rho_ps(G) += S(G) rhopsg(|G|) / omega for one
 // species, S(G) = sum_a exp(-i G.tau_a) (StructureFactor::update), complex interleaved
 __global__ void __launch_bounds__(256)
@@ -622,12 +702,10 @@ extern "C" int qbh_species_eval(qbh_species sp, int which, int n, const double* 
   return QBH_OK;
 }
 
-extern "C" int qbh_species_twnl(qbh_species sp, qbh_basis basis, const double* A,
-                                double* twnl_dev) {
-  if (!sp || !basis || !twnl_dev) return QBH_ERR_ARG;
-  Basis* b = basis_of(basis);
-  Ctx* c = ctx_of_basis(basis);
-  hipSetDevice(c->device);
+// twnl rows of species sp at |k+G-A| on basis b (grad 0: rows x ngw into out), or their
+// gradient with respect to k+G-A (grad 1: out + d * dstride, d = x, y, z), on c's stream
+int qbh::species_table(qbh_species sp, Ctx* c, Basis* b, const double* A, int grad, double* out,
+                       long dstride) {
   const double* tab = species_dev(sp, c->device);
   if (!tab) {
     c->last_error = "species table allocation failed";
@@ -641,20 +719,22 @@ extern "C" int qbh_species_twnl(qbh_species sp, qbh_basis basis, const double* A
     if (l == sp->llocal) continue;
     const double* py = tab + 3L * n + 2L * n * l;
     const double* py2 = py + n;
-    double* out = twnl_dev + row * ngw;
-    switch (l) {
-      case 0:
-        k_twnl<0><<<grid, 256, 0, c->stream>>>(ngw, b->kpg2, b->kpgx, ax, ay, az, tab, py, py2, n, out);
-        break;
-      case 1:
-        k_twnl<1><<<grid, 256, 0, c->stream>>>(ngw, b->kpg2, b->kpgx, ax, ay, az, tab, py, py2, n, out);
-        break;
-      case 2:
-        k_twnl<2><<<grid, 256, 0, c->stream>>>(ngw, b->kpg2, b->kpgx, ax, ay, az, tab, py, py2, n, out);
-        break;
-      default:
-        k_twnl<3><<<grid, 256, 0, c->stream>>>(ngw, b->kpg2, b->kpgx, ax, ay, az, tab, py, py2, n, out);
-        break;
+    double* o = out + row * ngw;
+    hipStream_t st = c->stream;
+    if (grad) {
+      switch (l) {
+        case 0: k_dtwnl<0><<<grid, 256, 0, st>>>(ngw, b->kpg2, b->kpgx, ax, ay, az, tab, py, py2, n, o, dstride); break;
+        case 1: k_dtwnl<1><<<grid, 256, 0, st>>>(ngw, b->kpg2, b->kpgx, ax, ay, az, tab, py, py2, n, o, dstride); break;
+        case 2: k_dtwnl<2><<<grid, 256, 0, st>>>(ngw, b->kpg2, b->kpgx, ax, ay, az, tab, py, py2, n, o, dstride); break;
+        default: k_dtwnl<3><<<grid, 256, 0, st>>>(ngw, b->kpg2, b->kpgx, ax, ay, az, tab, py, py2, n, o, dstride); break;
+      }
+    } else {
+      switch (l) {
+        case 0: k_twnl<0><<<grid, 256, 0, st>>>(ngw, b->kpg2, b->kpgx, ax, ay, az, tab, py, py2, n, o); break;
+        case 1: k_twnl<1><<<grid, 256, 0, st>>>(ngw, b->kpg2, b->kpgx, ax, ay, az, tab, py, py2, n, o); break;
+        case 2: k_twnl<2><<<grid, 256, 0, st>>>(ngw, b->kpg2, b->kpgx, ax, ay, az, tab, py, py2, n, o); break;
+        default: k_twnl<3><<<grid, 256, 0, st>>>(ngw, b->kpg2, b->kpgx, ax, ay, az, tab, py, py2, n, o); break;
+      }
     }
     row += 2 * l + 1;
   }
@@ -664,6 +744,26 @@ extern "C" int qbh_species_twnl(qbh_species sp, qbh_basis basis, const double* A
     return QBH_ERR_DEVICE;
   }
   return QBH_OK;
+}
+
+int qbh::species_nproj(qbh_species sp) { return nproj_of(sp); }
+
+extern "C" int qbh_species_twnl(qbh_species sp, qbh_basis basis, const double* A,
+                                double* twnl_dev) {
+  if (!sp || !basis || !twnl_dev) return QBH_ERR_ARG;
+  Basis* b = basis_of(basis);
+  Ctx* c = ctx_of_basis(basis);
+  hipSetDevice(c->device);
+  return species_table(sp, c, b, A, 0, twnl_dev, 0);
+}
+
+extern "C" int qbh_species_dtwnl(qbh_species sp, qbh_basis basis, const double* A,
+                                 double* dtwnl_dev) {
+  if (!sp || !basis || !dtwnl_dev) return QBH_ERR_ARG;
+  Basis* b = basis_of(basis);
+  Ctx* c = ctx_of_basis(basis);
+  hipSetDevice(c->device);
+  return species_table(sp, c, b, A, 1, dtwnl_dev, (long)nproj_of(sp) * b->g.ngw);
 }
 
 extern "C" int qbh_local_tables(qbh_basis vbasis, int nsp, const qbh_species* sps, const int* na,

@@ -9,17 +9,23 @@
 //   update: v_b = update_vhxc(rho(psi~)) + v_ext; psi_{n+1} = T(dt/2) psi(t+dt/2) under
 //     H[v_b, A_{n+1}] (ExponentialWavefunctionStepper.cc:243-276).
 // Objective J = sum_n w_n e.j(psi_n, A_n), j_d = -sum_b f_b sum_G (k+G)_d |c|^2 + A_d N_e.
-// The KB radial tables stay at |k+G|; the projector phases at k+G-A cancel in
-// beta D beta^H, so dH/dA_d = -(k+G-A)_d on the diagonal.
+// A enters H through 0.5 |k+G-A|^2 and, when the Hamiltonian's projector tables come from
+// species (qbh_ham_set_species), through twnl re-tabled at |k+G-A| on every change of A
+// (NonLocalPotential::update_twnl under a vector potential); the projector phases at k+G-A
+// cancel in beta D beta^H.  So dH/dA_d = -(k+G-A)_d on the diagonal plus, with species,
+// sum_p |d_d beta_p> D_p <beta_p| + h.c., d_d beta_p = -(-i)^l (d twnl_p / dq_d) e^{-i(k+G-A).tau}.
 //
 // Backward: exact derivative of that discrete map (Wirtinger gradients, dJ = 2 Re <g, dpsi>).
 // psi is checkpointed every `checkpoint` steps; each segment is recomputed with its Krylov
 // vectors H^k x stored, then walked in reverse:
 //   T^H = Taylor with dt -> -dt; the local-potential gradient of one exponential is
 //   mu(r) = (2/N) Re sum_b sum_k conj(z_k(r)) U_k(r), z_k = H^k lambda,
-//   U_k = sum_m alpha_{k+m+1} u_m; dJ/dA_d = -2 Re sum conj(z_k) (k+G-A)_d U_k;
+//   U_k = sum_m alpha_{k+m+1} u_m; dJ/dA_d = -2 Re sum conj(z_k) (k+G-A)_d U_k, plus with
+//   species -2 Re sum_{p,b} D_p [conj(dF^z_d) F^U + conj(F^z) dF^U_d], F = anl^H x and
+//   dF_d = danl_d^H x, danl_d built like anl from d twnl / dq_d;
 //   through v = V_Hxc[rho]: nu = f_xc(rho) mu + K_H mu, lambda += (f_b N/Omega) V_loc[nu] psi.
 #include <hip/hip_runtime.h>
+#include <rocblas/rocblas.h>
 
 #include <algorithm>
 #include <cmath>
@@ -155,6 +161,30 @@ k_crossk_part(int ngw, int nst, const double* __restrict__ kpgx, double A0, doub
   }
 }
 
+// partials of s_d = Re sum_{p,b} fac_p [conj(Fz[(1+d)P + p, b]) FU[p, b] + conj(Fz[p, b])
+// FU[(1+d)P + p, b]] for d = 0..2; Fz, FU complex with leading dimension 4P
+__global__ void __launch_bounds__(256)
+k_nl_dA_part(int P, int nst, const double* __restrict__ fac, const double* __restrict__ Fz,
+             const double* __restrict__ FU, double* __restrict__ part) {
+  __shared__ double red[256];
+  double s[3] = {0.0, 0.0, 0.0};
+  const long tot = (long)P * nst;
+  const long ld = 4L * P;
+  for (long i = blockIdx.x * 256L + threadIdx.x; i < tot; i += (long)gridDim.x * 256) {
+    const long p = i % P, b = i / P;
+    const long o = 2 * (b * ld + p);
+    const double zr = Fz[o], zi = Fz[o + 1], ur = FU[o], ui = FU[o + 1];
+    for (int d = 0; d < 3; ++d) {
+      const long od = o + 2L * (1 + d) * P;
+      s[d] += fac[p] * ((Fz[od] * ur + Fz[od + 1] * ui) + (zr * FU[od] + zi * FU[od + 1]));
+    }
+  }
+  for (int d = 0; d < 3; ++d) {
+    const double a = block_sum(s[d], red);
+    if (threadIdx.x == 0) part[d * gridDim.x + blockIdx.x] = a;
+  }
+}
+
 // mu[r] += scale * sum_{b < nb} Re(conj(fz_b(r)) fU_b(r))
 __global__ void k_cross_acc(long N, int nb, double scale, const double* __restrict__ fz,
                             const double* __restrict__ fU, double* __restrict__ mu) {
@@ -258,6 +288,11 @@ struct qbh_td_s {
          *grids = nullptr;
   double *va = nullptr, *vb_buf = nullptr, *rho_a = nullptr, *rho_b = nullptr;
   int nb_cross = 1;
+  // A-derivative of species-tabled projectors: rows at A and their q-gradient (3 blocks),
+  // [anl | danl_x | danl_y | danl_z] (ngw x 4P complex), the products with z_k and U_k
+  bool nl_adj = false;
+  int nrows = 0;
+  double *tw = nullptr, *dtw = nullptr, *anl4 = nullptr, *fz4 = nullptr, *fu4 = nullptr;
   size_t blk() const { return 2 * (size_t)ngw * nst; }
   size_t seg_step() const { return (3 * (size_t)order + 1) * blk() + 4 * (size_t)N; }
 };
@@ -341,6 +376,35 @@ static int cross(qbh_td_s* t, const double* zk, const double* u, const double* A
   a[0] = 1.0;
   for (int N = 1; N <= o; ++N) a[N] = a[N - 1] * std::complex<double>(0.0, -0.5 * t->dt / N);
   qbh_basis hb = reinterpret_cast<qbh_basis>(t->h->b_handle);
+  Ham* h = t->h;
+  const int P = h->P;
+  const bool nl = t->nl_adj && (whole_h_flags(t->flags) & QBH_NONLOCAL);
+  rocblas_handle bh = nullptr;
+  if (nl) {
+    // tables at this exponential's A: twnl rows, their gradient, anl and danl_d
+    const long rs = (long)t->nrows * t->ngw;
+    long row0 = 0;
+    for (qbh_species sp : h->sps) {
+      if (int r = species_table(sp, cx, h->b, A, 0, t->tw + row0 * t->ngw, 0)) return r;
+      if (int r = species_table(sp, cx, h->b, A, 1, t->dtw + row0 * t->ngw, rs)) return r;
+      row0 += species_nproj(sp);
+    }
+    const size_t cs = 2 * (size_t)t->ngw * P;
+    if (int r = build_anl_c64(h, t->tw, A, t->anl4)) return r;
+    for (int d = 0; d < 3; ++d)
+      if (int r = build_anl_c64(h, t->dtw + d * rs, A, t->anl4 + (1 + d) * cs)) return r;
+    void* hv = nullptr;
+    if (int r = blas_handle(cx, &hv)) return tfail(cx, r, "rocBLAS handle creation failed");
+    bh = static_cast<rocblas_handle>(hv);
+  }
+  auto products = [&](const double* x, double* out) {
+    const rocblas_double_complex one(1.0, 0.0), zero(0.0, 0.0);
+    return rocblas_zgemm(bh, rocblas_operation_conjugate_transpose, rocblas_operation_none,
+                         4 * P, t->nst, t->ngw, &one,
+                         reinterpret_cast<const rocblas_double_complex*>(t->anl4), t->ngw,
+                         reinterpret_cast<const rocblas_double_complex*>(x), t->ngw, &zero,
+                         reinterpret_cast<rocblas_double_complex*>(out), 4 * P);
+  };
   const int nb = t->nb_cross;
   double* fz = t->grids;
   double* fU = t->grids + 2 * (size_t)nb * t->N;
@@ -357,6 +421,13 @@ static int cross(qbh_td_s* t, const double* zk, const double* u, const double* A
     k_crossk_part<<<NBLK, 256, 0, cx->stream>>>(t->ngw, t->nst, t->h->b->kpgx, A[0], A[1], A[2],
                                                z, t->ubuf, t->part);
     k_final<<<1, 256, 0, cx->stream>>>(NBLK, 3, t->part, -2.0, 1, gA3);
+    if (nl) {
+      if (products(z, t->fz4) != rocblas_status_success ||
+          products(t->ubuf, t->fu4) != rocblas_status_success)
+        return tfail(cx, QBH_ERR_BLAS, "td projector products failed");
+      k_nl_dA_part<<<NBLK, 256, 0, cx->stream>>>(P, t->nst, h->col_fac, t->fz4, t->fu4, t->part);
+      k_final<<<1, 256, 0, cx->stream>>>(NBLK, 3, t->part, -2.0, 1, gA3);
+    }
     for (int b0 = 0; b0 < t->nst; b0 += nb) {
       const int m = std::min(nb, t->nst - b0);
       if (int r = qbh_fft_backward(hb, z + 2 * (size_t)b0 * t->ngw, t->ngw, m, 0, fz, 0)) return r;
@@ -414,6 +485,15 @@ static int ensure_memory(qbh_td_s* t) {
                           nck * B, t->ckpt * t->seg_step(), B, B, B, B, t->order * B, B, N, N, N,
                           2 * N, 2 * (size_t)t->ngv, 4 * (size_t)t->nb_cross * N, N, N, N, N};
   for (size_t s : sizes) add(s);
+  Ham* h = t->h;
+  t->nl_adj = !h->sps.empty() && h->P > 0;
+  t->nrows = 0;
+  for (qbh_species sp : h->sps) t->nrows += species_nproj(sp);
+  const size_t rows_d = (size_t)t->nrows * t->ngw;
+  const size_t tabs[] = {rows_d, 3 * rows_d, 8 * (size_t)t->ngw * h->P,
+                         8 * (size_t)h->P * t->nst, 8 * (size_t)h->P * t->nst};
+  if (t->nl_adj)
+    for (size_t s : tabs) add(s);
   if (need > t->mem_bytes) {
     if (t->mem) hipFree(t->mem);
     t->mem = nullptr;
@@ -454,6 +534,13 @@ static int ensure_memory(qbh_td_s* t) {
   t->vb_buf = carve(p, N);
   t->rho_a = carve(p, N);
   t->rho_b = carve(p, N);
+  if (t->nl_adj) {
+    t->tw = carve(p, tabs[0]);
+    t->dtw = carve(p, tabs[1]);
+    t->anl4 = carve(p, tabs[2]);
+    t->fz4 = carve(p, tabs[3]);
+    t->fu4 = carve(p, tabs[4]);
+  }
   hipMemcpyAsync(t->fj, t->jfac.data(), t->nst * sizeof(double), hipMemcpyHostToDevice,
                  t->c->stream);
   return QBH_OK;

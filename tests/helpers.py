@@ -60,3 +60,21 @@ def make_gpu_ham(ctx, basis, sysm, with_proj=True):
         h.set_projectors([sysm.na], [len(sysm.lproj)], sysm.lproj, sysm.wt, sysm.twnl.ravel(),
                          sysm.tau.ravel(), sysm.omega)
     return h
+
+
+def synthetic_spdf():
+    """Radial data of a KB species with s... f channels (lmax 3, local channel l = 0, so p,
+    d and f projectors), smooth functions on a 0.01 mesh: the arguments of
+    qball_amd.Species / oracle.species.Species."""
+    n = 900
+    r = np.arange(n) * 0.01
+    zv = 3.0
+    lmax, llocal = 3, 0
+    vps, phi = [], []
+    for l in range(lmax + 1):
+        v = -zv * np.where(r > 0, 1.0, 0.0) * np.minimum(1.0 / np.maximum(r, 1e-9), 3.0)
+        v = v + (1.5 - 0.2 * l) * np.exp(-(r / (1.0 + 0.1 * l)) ** 2)
+        vps.append(v)
+        phi.append((r ** (l + 1)) * np.exp(-0.8 * r * r))
+    return {"zval": zv, "lmax": lmax, "llocal": llocal, "deltar": 0.01, "vps": vps,
+            "phi": phi, "nquad": 0}

@@ -14,29 +14,13 @@ import os
 import numpy as np
 import pytest
 
-from helpers import from_dev, make_gpu_basis, oracle_for, real_to_dev, rel_err, to_dev
+from helpers import (from_dev, make_gpu_basis, oracle_for, real_to_dev, rel_err,
+                     synthetic_spdf, to_dev)
 
 pytestmark = pytest.mark.gpu
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 SI_XML = os.path.join(HERE, "golden", "Si_HSCV_LDA-1.0.xml")
-
-
-def _synthetic_spdf(rcps):
-    """A KB species with p, d and f projectors (lmax 3, local channel l = 0), smooth radial
-    functions on a 0.01 mesh."""
-    n = 900
-    r = np.arange(n) * 0.01
-    zv = 3.0
-    lmax, llocal = 3, 0
-    vps, phi = [], []
-    for l in range(lmax + 1):
-        v = -zv * np.where(r > 0, 1.0, 0.0) * np.minimum(1.0 / np.maximum(r, 1e-9), 3.0)
-        v = v + (1.5 - 0.2 * l) * np.exp(-(r / (1.0 + 0.1 * l)) ** 2)
-        vps.append(v)
-        phi.append((r ** (l + 1)) * np.exp(-0.8 * r * r))
-    return {"zval": zv, "lmax": lmax, "llocal": llocal, "deltar": 0.01, "vps": vps,
-            "phi": phi, "nquad": 0}
 
 
 @pytest.fixture(scope="module")
@@ -80,7 +64,7 @@ def test_twnl_spdf_matches_oracle(gpu_ctx):
     from oracle import species as S
     from qball_amd import Species
     from qball_amd.workload import make_system
-    d = _synthetic_spdf(1.0)
+    d = synthetic_spdf()
     ora = S.Species(d, 1.0)
     sp = Species(rcps=1.0, zval=d["zval"], lmax=d["lmax"], llocal=d["llocal"],
                  deltar=d["deltar"], vps=np.stack(d["vps"]), phi=np.stack(d["phi"]))
@@ -108,7 +92,7 @@ def test_local_tables_match_oracle(gpu_ctx, si_pair):
     vr = capi.rods_build(s.a, 4.0 * s.ecut, False)
     vb = Basis(gpu_ctx, False, vr["rod_h"], vr["rod_k"], vr["rod_lmin"], vr["rod_size"], s.np, s.b)
     _, gx = plane_waves(vr, s.b)
-    d = _synthetic_spdf(1.0)
+    d = synthetic_spdf()
     sp2 = Species(rcps=1.0, zval=d["zval"], lmax=d["lmax"], llocal=d["llocal"],
                   deltar=d["deltar"], vps=np.stack(d["vps"]), phi=np.stack(d["phi"]))
     ora2 = S.Species(d, 1.0)

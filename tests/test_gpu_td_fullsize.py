@@ -5,7 +5,11 @@ A0 = 0.01, objective the time-integrated dipole along x, psi stored every 20 ste
 
 Through the torch autograd function: dJ/dA0 and dJ/dv_ext (along one smooth direction)
 agree with central differences (h = 1e-5) of the FP64 forward to 1e-6 relative; the
-FP32-bulk form's J and gradients are compared with the FP64 ones."""
+FP32-bulk form's J and gradients are compared with the FP64 ones.  Two Hamiltonians: fixed
+analytic s+p projector tables with smooth Gaussian ionic inputs, and the Si HSCV LDA
+pseudopotential (projectors re-tabled at |k+G-A| every step, v_ion and rho_ps from its
+local part)."""
+import os
 import time
 
 import numpy as np
@@ -18,25 +22,43 @@ pytestmark = pytest.mark.gpu
 NSTEPS, DT, CHECKPOINT, H = 200, 0.05, 20, 1e-5
 
 
-@pytest.fixture(scope="module")
-def si64td():
+SI_XML = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "Si_HSCV_LDA-1.0.xml")
+
+
+@pytest.fixture(scope="module", params=["analytic", "pseudopotential"])
+def si64td(request):
     import torch
     from oracle.basis import Basis as OB, UnitCell
-    from qball_amd import Basis, Context, TDPropagator, capi
+    from qball_amd import Basis, Context, Species, TDPropagator, capi, local_tables
     from qball_amd.workload import make_system, td_ionic_inputs
     sysm = make_system(ncell=2, ecut_ry=25.0, nbands=128, seed=41, complex_basis=True)
     stream = torch.cuda.current_stream()
     ctx = Context(0, stream.cuda_stream)
     basis = make_gpu_basis(ctx, sysm)
-    ham = make_gpu_ham(ctx, basis, sysm)
     r = capi.rods_build(sysm.a, 4.0 * sysm.ecut, False)
     vb = Basis(ctx, False, r["rod_h"], r["rod_k"], r["rod_lmin"], r["rod_size"], sysm.np, sysm.b)
     vob = OB(UnitCell(sysm.a), 4.0 * sysm.ecut, force_complex=True)
     assert vob.size == vb.ngw
-    # ionic inputs of physical size (Gaussian ion charges neutral with the 256 electrons and a
-    # short-range local potential), so that dt * |H| stays inside the Taylor propagator's range
-    vion, rhops = td_ionic_inputs(sysm, r)
     cplx = lambda z: torch.from_numpy(np.ascontiguousarray(z).view(np.float64).copy()).cuda()
+    species = None
+    if request.param == "analytic":
+        ham = make_gpu_ham(ctx, basis, sysm)
+        # ionic inputs of physical size (Gaussian ion charges neutral with the 256 electrons
+        # and a short-range local potential), so that dt * |H| stays inside the Taylor
+        # propagator's range
+        vion, rhops = td_ionic_inputs(sysm, r)
+    else:
+        species = Species(SI_XML, rcps=1.5)
+        ham = make_gpu_ham(ctx, basis, sysm, with_proj=False)
+        ham.set_species([species], [sysm.na], sysm.tau, sysm.omega)
+        tau_d = torch.from_numpy(np.ascontiguousarray(sysm.tau).ravel().copy()).cuda()
+        vg = torch.zeros(2 * vb.ngw, dtype=torch.float64, device="cuda")
+        rg = torch.zeros_like(vg)
+        local_tables(vb, [species], [sysm.na], tau_d.data_ptr(), sysm.omega, vg.data_ptr(),
+                     rg.data_ptr())
+        ctx.sync()
+        vion = vg.cpu().numpy().view(np.complex128)
+        rhops = rg.cpu().numpy().view(np.complex128)
     psi = cplx(np.ascontiguousarray(sysm.psi.T))
     basis.gram(psi.data_ptr(), basis.ngw, sysm.nst)        # orthonormal start (FP64, GPU)
     n0, n1, n2 = sysm.np
@@ -49,10 +71,13 @@ def si64td():
                vext=torch.from_numpy(vext.copy()).cuda(), dv=torch.from_numpy(dv.copy()).cuda())
     td = TDPropagator(ham, vb, sysm.nst, occ, 1.0, sysm.omega, dev["vion"].data_ptr(),
                       dev["rhops"].data_ptr())
-    print("Si64 TD: ngw", basis.ngw, "bands", sysm.nst, "grid", sysm.np, flush=True)
+    print("Si64 TD (", request.param, "): ngw", basis.ngw, "bands", sysm.nst, "grid", sysm.np,
+          flush=True)
     yield dict(sysm=sysm, ctx=ctx, basis=basis, td=td, dev=dev)
     td.close()
     ham.close()
+    if species is not None:
+        species.close()
     vb.close()
     basis.close()
     ctx.close()
