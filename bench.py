@@ -2,7 +2,7 @@
 """Hψ throughput benchmark: band-applications/s of a full Hamiltonian application
 (non-local KB projectors + kinetic + local potential via the G<->r transforms) on diamond
 Si512 at Γ (a = 41.04 bohr, ecut 25 Ry: 72919 plane waves, 1024 bands, 140^3 grid,
-512 x 4 projectors), one MI355X.
+512 x 4 projectors of the Si HSCV LDA pseudopotential), one MI355X.
 
 One step = one Hψ over all 1024 bands (qbh_hpsi with QBH_ALL), inputs resident in HBM.
 The projector table is generated on the first (warm-up) application and reused, as the
@@ -25,6 +25,7 @@ import numpy as np
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
+SI_XML = os.path.join(ROOT, "tests", "golden", "Si_HSCV_LDA-1.0.xml")
 PEAK_HBM_GBS = 8000.0
 PEAK_FP64_TFLOPS = 78.6
 PEAK_FP32_TFLOPS = 157.3
@@ -50,6 +51,10 @@ def parse():
     p.add_argument("--no-large", action="store_true", help="skip the Al500 large-cell section")
     p.add_argument("--no-td", action="store_true",
                    help="skip the Si64 TDDFT forward/adjoint section (differentiation form)")
+    p.add_argument("--no-scf", action="store_true",
+                   help="skip the Si512 SCF section (precision form against FP64)")
+    p.add_argument("--scf-cycles", type=int, default=20)
+    p.add_argument("--scf-nite", type=int, default=3)
     p.add_argument("--batch-streams", type=int, default=4,
                    help="contexts (HIP streams) the 12 batch cells are spread over")
     p.add_argument("--no-tune", action="store_true",
@@ -99,7 +104,7 @@ def main():
     if args.calibrate_gib > 0:
         calibrate(args.calibrate_gib)
         return
-    from qball_amd import Basis, Context, Hamiltonian, QBH_ALL, QBH_FP32
+    from qball_amd import Basis, Context, Hamiltonian, QBH_ALL, QBH_FP32, Species, auto_rcps
     from qball_amd.devmem import DeviceArray, synchronize
     from qball_amd.workload import make_system
 
@@ -114,8 +119,10 @@ def main():
     ham = Hamiltonian(ctx, basis)
     v = DeviceArray.from_numpy(sysm.v)
     ham.set_vloc(v.ptr)
-    ham.set_projectors([sysm.na], [4], sysm.lproj, sysm.wt, sysm.twnl.ravel(), sysm.tau.ravel(),
-                       sysm.omega)
+    # the Si HSCV LDA pseudopotential with the Ewald width AtomSet::set_rcps picks; the
+    # projector tables are made on the device (update_twnl)
+    species = Species(SI_XML, rcps=auto_rcps(4.0, sysm.a, sysm.ecut))
+    ham.set_species([species], [sysm.na], sysm.tau, sysm.omega)
     ngw, nst = sysm.ngw, sysm.nst
     psi = DeviceArray.from_numpy(np.ascontiguousarray(sysm.psi.T).view(np.float64).ravel())
     del sysm.psi
@@ -317,6 +324,10 @@ def main():
                 "tuning": sched32, "kernels_sequential_builtin": k32}
     del h64
 
+    scf = None
+    if not args.no_scf:
+        scf = scf_bench(args, ctx, basis, ham, sysm, species, psi)
+        log("scf section done")
     batch = None
     if not args.no_batch:
         batch = batch_bench(args)
@@ -348,7 +359,8 @@ def main():
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f64",
-        "data": "synthetic (seeded psi, smooth random v_loc, analytic s+p KB projectors)",
+        "data": "synthetic (seeded psi, smooth random v_loc; KB projectors of the Si HSCV LDA "
+                "pseudopotential)",
         "config": {"workload": f"Si{sysm.na} diamond Γ, a={10.26 * args.ncell:.2f} bohr, ecut 25 Ry, "
                                f"{ngw} PWs, {nst} bands, grid {sysm.np[0]}^3, {4 * sysm.na} KB projectors",
                    "global_batch": nst, "parallelism": "single GPU"},
@@ -356,6 +368,7 @@ def main():
         "fft_stages": fft,
         "cpu_baseline": cpu,
         "fp32_bulk": fp32,
+        "scf": scf,
         "batching": batch,
         "large_cell": large,
         "differentiation": td,
@@ -369,6 +382,70 @@ def main():
     ham.close()
     basis.close()
     ctx.close()
+
+
+def scf_bench(args, ctx, basis, ham, sysm, species, psi0):
+    """The precision form at its own size: the BOSampleStepper SCF loop (PSDA + Anderson /
+    Kerker charge mixing, qbh_scf_run) on Si512 with the Si HSCV LDA pseudopotential (v_ion,
+    rho_ps from its local part, Ewald esr - eself added to etotal), from the benchmark's
+    orthonormal start, run twice: all Hψ in FP64, and FP32-bulk Hψ with the last inner
+    iteration of every cycle in FP64.  Reports etotal of both, ΔE per atom, the density L1
+    difference and the non-local force difference of the final states, and times."""
+    from qball_amd import QBH_ALL, QBH_FP32, Basis, capi, ewald_esr, local_tables
+    from qball_amd.devmem import DeviceArray, synchronize
+    ngw, nst = sysm.ngw, sysm.nst
+    vr = capi.rods_build(sysm.a, 4.0 * sysm.ecut, True)
+    vb = Basis(ctx, True, vr["rod_h"], vr["rod_k"], vr["rod_lmin"], vr["rod_size"], sysm.np,
+               sysm.b)
+    tau = DeviceArray.from_numpy(np.ascontiguousarray(sysm.tau).ravel())
+    vion = DeviceArray(16 * vb.ngw)
+    rhops = DeviceArray(16 * vb.ngw)
+    local_tables(vb, [species], [sysm.na], tau.ptr, sysm.omega, vion.ptr, rhops.ptr)
+    esr = ewald_esr(sysm.a, [species], [sysm.na], sysm.tau)
+    eself = sysm.na * species.eself
+    occ = np.full(nst, 2.0)
+    start = psi0.to_numpy()
+    psi = DeviceArray(psi0.nbytes)
+    N = int(np.prod(sysm.np))
+    rho = DeviceArray(8 * N)
+    runs = {}
+    for name, fi in (("fp64", QBH_ALL), ("precision", QBH_ALL | QBH_FP32)):
+        psi.upload(start)
+        synchronize()
+        t0 = time.perf_counter()
+        hist = ham.scf_run(vb, psi.ptr, ngw, nst, occ, sysm.omega, vion.ptr, rhops.ptr,
+                           args.scf_cycles, args.scf_nite, sysm.ecut, flags_inner=fi,
+                           flags_last=QBH_ALL)
+        synchronize()
+        t1 = time.perf_counter()
+        rho.zero()
+        basis.density(psi.ptr, ngw, nst, occ, 1.0, sysm.omega, rho.ptr)
+        ctx.sync()
+        f = ham.nl_forces(psi.ptr, ngw, nst, occ)
+        et = hist[:, -1, 0] + esr - eself
+        runs[name] = {"etotal_Ha": float(et[-1]), "etotal_per_cycle_Ha": [float(x) for x in et],
+                      "last_cycle_change_Ha": float(et[-1] - et[-2]) if len(et) > 1 else None,
+                      "terms_Ha": dict(zip(("ekin", "enl", "eps", "ehart", "exc"),
+                                           [float(x) for x in hist[-1, -1, 1:6]])),
+                      "seconds": t1 - t0, "rho": rho.to_numpy(), "fnl": f}
+        log(f"scf {name}: etotal {et[-1]:.10f} Ha in {t1 - t0:.2f} s "
+            f"(last cycle change {runs[name]['last_cycle_change_Ha']:.3e})")
+    a, b = runs["fp64"], runs["precision"]
+    out = {"workload": f"Si{sysm.na} Γ, Si HSCV LDA (rcps {species.rcps:.2f}), {ngw} PWs, "
+                       f"{nst} bands, grid {sysm.np[0]}^3, {args.scf_cycles} SCF cycles x "
+                       f"{args.scf_nite} PSDA iterations, Anderson ndim 3, Kerker rc 10",
+           "esr_Ha": esr, "eself_Ha": eself,
+           "dE_per_atom_Ha": abs(b["etotal_Ha"] - a["etotal_Ha"]) / sysm.na,
+           "density_L1_electrons": float(np.abs(b["rho"] - a["rho"]).sum() * sysm.omega / N),
+           "nl_force_max_abs_diff_Ha_per_bohr": float(np.max(np.abs(b["fnl"] - a["fnl"]))),
+           "nl_force_max_abs_Ha_per_bohr": float(np.max(np.abs(a["fnl"]))),
+           "speedup": a["seconds"] / b["seconds"]}
+    for k in ("fp64", "precision"):
+        out[k] = {x: y for x, y in runs[k].items() if x not in ("rho", "fnl")}
+    for d in (psi, rho, tau, vion, rhops):
+        d.close()
+    vb.close()
+    return out
 
 
 def load_traffic():
@@ -539,10 +616,13 @@ def td_bench(args):
     waves, 128 bands, 72^3 grid), 200 ETRS steps of dt = 0.05 under A(t) = A0 sin(0.3 t)
     exp(-(t - t0)^2 / 2 tau^2) along x (A0 = 0.01), objective the time-integrated dipole
     along x, psi stored every 20 steps: forward and adjoint (backward, with the segment
-    recomputation) times in FP64 and FP32-bulk, and the FP32-bulk J and dJ/dA0 against FP64."""
-    from qball_amd import QBH_FP32, Basis, Context, Hamiltonian, TDPropagator, capi
+    recomputation) times in FP64 and FP32-bulk, and the FP32-bulk J and dJ/dA0 against FP64.
+    The Si HSCV LDA pseudopotential: projectors re-tabled at |k+G-A| at every step, v_ion and
+    rho_ps from its local part."""
+    from qball_amd import (QBH_FP32, Basis, Context, Hamiltonian, Species, TDPropagator,
+                           auto_rcps, capi, local_tables)
     from qball_amd.devmem import DeviceArray, synchronize
-    from qball_amd.workload import dipole_weights, field_shape, make_system, td_ionic_inputs
+    from qball_amd.workload import dipole_weights, field_shape, make_system
     nsteps, dt, ckpt, A0 = 200, 0.05, 20, 0.01
     s = make_system(ncell=2, ecut_ry=25.0, nbands=128, seed=41, complex_basis=True)
     ctx = Context(0, None)
@@ -551,13 +631,14 @@ def td_bench(args):
     ham = Hamiltonian(ctx, basis)
     v = DeviceArray.from_numpy(s.v)
     ham.set_vloc(v.ptr)
-    ham.set_projectors([s.na], [len(s.lproj)], s.lproj, s.wt, s.twnl.ravel(), s.tau.ravel(),
-                       s.omega)
+    species = Species(SI_XML, rcps=auto_rcps(4.0, s.a, s.ecut))
+    ham.set_species([species], [s.na], s.tau, s.omega)
     vr = capi.rods_build(s.a, 4.0 * s.ecut, False)
     vb = Basis(ctx, False, vr["rod_h"], vr["rod_k"], vr["rod_lmin"], vr["rod_size"], s.np, s.b)
-    vion, rhops = td_ionic_inputs(s, vr)
-    dvion = DeviceArray.from_numpy(np.ascontiguousarray(vion).view(np.float64))
-    drhops = DeviceArray.from_numpy(np.ascontiguousarray(rhops).view(np.float64))
+    tau = DeviceArray.from_numpy(np.ascontiguousarray(s.tau).ravel())
+    dvion = DeviceArray(16 * vb.ngw)
+    drhops = DeviceArray(16 * vb.ngw)
+    local_tables(vb, [species], [s.na], tau.ptr, s.omega, dvion.ptr, drhops.ptr)
     psi = DeviceArray.from_numpy(np.ascontiguousarray(s.psi.T).view(np.float64).ravel())
     basis.gram(psi.ptr, s.ngw, s.nst)
     n0, n1, n2 = s.np
@@ -569,7 +650,8 @@ def td_bench(args):
     td = TDPropagator(ham, vb, s.nst, occ, 1.0, s.omega, dvion.ptr, drhops.ptr)
     shape = field_shape(nsteps, dt, omega=0.3)
     w = dipole_weights(nsteps, dt)
-    out = {"workload": f"Si{s.na} complex basis, {s.ngw} PWs, {s.nst} bands, grid {n0}^3, "
+    out = {"workload": f"Si{s.na} complex basis, Si HSCV LDA (projectors re-tabled at "
+                       f"|k+G-A| every step), {s.ngw} PWs, {s.nst} bands, grid {n0}^3, "
                        f"{nsteps} ETRS steps of dt={dt}, checkpoint every {ckpt} steps"}
     for name, fl in (("fp64", 0), ("fp32_bulk", QBH_FP32)):
         synchronize()
@@ -588,6 +670,7 @@ def td_bench(args):
                                               / abs(out["fp64"]["dJ_dA0"]))
     td.close()
     ham.close()
+    species.close()
     vb.close()
     basis.close()
     ctx.close()
@@ -606,6 +689,7 @@ def cpu_baseline(sysm, local_bands, psi_dev, ngw, nst):
     from threadpoolctl import threadpool_limits
     from oracle import fft as offt
     from oracle import hpsi as oh
+    from oracle import species as osp
     from oracle.basis import Basis as OB, FFTMaps, UnitCell
     cores = max(1, min(16, len(os.sched_getaffinity(0))))
     # keep large blocks on the heap and freed memory mapped, so the pass-to-pass buffers
@@ -617,7 +701,9 @@ def cpu_baseline(sysm, local_bands, psi_dev, ngw, nst):
     ob = OB(UnitCell(sysm.a), sysm.ecut)
     maps = FFTMaps(ob, *sysm.np)
     plan = offt.PrunedPlan(maps)
-    sp = oh.Projectors(sysm.lproj, sysm.wt, sysm.twnl, sysm.tau)
+    ora = osp.Species(osp.read_species_xml(SI_XML), osp.auto_rcps(4.0, sysm.a, sysm.ecut))
+    rows, ls, wt = osp.twnl_table(ora, ob.kpgx)
+    sp = oh.Projectors(ls, wt, rows, sysm.tau)
     c = psi_dev.to_numpy().view(np.complex128).reshape(nst, ngw).T.copy()
     nb = min(nst, max(2, 2 * (local_bands // 2)))
     workers = offt.WORKERS
@@ -626,8 +712,7 @@ def cpu_baseline(sysm, local_bands, psi_dev, ngw, nst):
         with threadpool_limits(limits=cores):
             warm = np.zeros((ngw, 4), dtype=np.complex128)
             oh.rs_mul_add_batched(maps, c[:, :4], sysm.v, warm, batch=1, plan=plan)
-            oh.nonlocal_energy(ob, [oh.Projectors(sysm.lproj, sysm.wt, sysm.twnl,
-                                                   sysm.tau[:min(128, sysm.na)])],
+            oh.nonlocal_energy(ob, [oh.Projectors(ls, wt, rows, sysm.tau[:min(128, sysm.na)])],
                                c[:, :64], sysm.occ[:64], sysm.omega, cp=np.zeros_like(c[:, :64]))
             del warm
             cp = np.zeros((ngw, nb), dtype=np.complex128)

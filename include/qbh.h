@@ -68,9 +68,11 @@ int qbh_ctx_timing_report(qbh_ctx ctx, char* buf, int cap);
 int qbh_ctx_library_info(qbh_ctx ctx, char* buf, int cap);
 /* Stream bandwidth of the context's device (no counterpart in the reference; diagnostics):
  * grid-stride read-only, write-only and copy kernels over two temporary buffers of `bytes`
- * (1 MiB .. 64 GiB), best of `reps` timed launches each; gbps[0..2] = read, write and copy
- * rates in GB/s, the copy rate counting read plus written bytes.  The practical ceiling
- * for the HBM rate of a streaming pass.  Synchronises the stream. */
+ * (1 MiB .. 64 GiB), each the best over 1024, 4096 and 16384 workgroups with 1, 4 or 8
+ * independent 16-byte accesses in flight per thread and over `reps` timed launches;
+ * gbps[0..2] = read, write and copy rates in GB/s, the copy rate counting read plus
+ * written bytes.  The practical ceiling for the HBM rate of a streaming pass.
+ * Synchronises the stream. */
 int qbh_ctx_stream_bandwidth(qbh_ctx ctx, double bytes, int reps, double* gbps);
 
 /* ---- plane-wave basis ------------------------------------------------------------------
@@ -197,6 +199,12 @@ int qbh_update_vhxc(qbh_basis vbasis, const double* rhor_dev, const double* rhog
 int qbh_species_create(double zval, int lmax, int llocal, double deltar, int npts,
                        const double* vps, const double* phi, double rcps, qbh_species* out);
 int qbh_species_create_xml(const char* path, double rcps, qbh_species* out);
+/* The Gaussian width AtomSet::set_rcps (AtomSet.cc:1102-1163) picks for a species of
+ * valence zval in the cell a (3 x 3 host, rows = cell vectors) at the wavefunction cutoff
+ * ecut (Hartree): the smallest multiple of 0.05 above 0.3 whose reciprocal-space tail is
+ * below 1e-7 rcps / zval, capped at min_wsdist / 2 pi and reduced until the charge outside
+ * the cell is below 1e-8. */
+int qbh_species_auto_rcps(double zval, const double* a, double ecut, double* rcps);
 int qbh_species_destroy(qbh_species sp);
 /* zval, rcps, eself = zval^2 / (sqrt(2 pi) rcps) (Species.h:159-160), the number of KB
  * projectors (sum of 2l+1 over l != llocal) and, per projector in the order of

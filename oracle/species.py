@@ -437,6 +437,21 @@ def min_wsdist(a):
     return float(np.min(np.sqrt(2.0 * an2h)))
 
 
+def auto_rcps(zval, a, ecut):
+    """AtomSet::set_rcps (AtomSet.cc:1102-1163): the Gaussian width for valence zval in
+    cell a at the wavefunction cutoff ecut (Hartree)."""
+    ellmin = min_wsdist(a)
+    r = 0.3
+    recipmax = ellmin / (1.0 * 2.0 * math.pi)
+    while (zval / math.sqrt(math.pi)) * erfc(r * math.sqrt(ecut)) > 1.0e-7 * r / zval:
+        r += 0.05
+    if r > recipmax:
+        r = recipmax
+    while zval * erfc(ellmin / (math.sqrt(2.0) * r)) > 1.0e-8:
+        r -= 0.05
+    return r
+
+
 def fold_in_ws(v, a):
     """UnitCell::fold_in_ws (UnitCell.cc:251-281)."""
     an, an2h = ws_planes(a)

@@ -7,8 +7,9 @@ ABI, used by the tests and the benchmark; it refuses to run without the built li
 """
 from .capi import (QBH_ALL, QBH_ACCUMULATE, QBH_FP32, QBH_KINETIC, QBH_LOCAL, QBH_NONLOCAL,
                    Basis, Batch, Context, Hamiltonian, QbhError, Species, TDPropagator,
-                   ewald_esr, factorizable, lib, local_tables, rods_build)
+                   auto_rcps, ewald_esr, factorizable, lib, local_tables, rods_build)
 
 __all__ = ["QBH_ALL", "QBH_ACCUMULATE", "QBH_FP32", "QBH_KINETIC", "QBH_LOCAL",
            "QBH_NONLOCAL", "Basis", "Batch", "Context", "Hamiltonian", "QbhError", "Species",
-           "TDPropagator", "ewald_esr", "factorizable", "lib", "local_tables", "rods_build"]
+           "TDPropagator", "auto_rcps", "ewald_esr", "factorizable", "lib", "local_tables",
+           "rods_build"]

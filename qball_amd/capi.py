@@ -90,6 +90,7 @@ SIGNATURES = {
                                     _c_double, ctypes.POINTER(_vp)]),
     "qbh_species_create_xml": (_c_int, [ctypes.c_char_p, _c_double, ctypes.POINTER(_vp)]),
     "qbh_species_destroy": (_c_int, [_vp]),
+    "qbh_species_auto_rcps": (_c_int, [_c_double, _dp, _c_double, _dp]),
     "qbh_species_info": (_c_int, [_vp, _dp, _dp, _dp, _ip, _ip, _dp]),
     "qbh_species_eval": (_c_int, [_vp, _c_int, _c_int, _dp, _dp, _dp]),
     "qbh_species_twnl": (_c_int, [_vp, _vp, _dp, _vp]),
@@ -358,6 +359,15 @@ class Species:
         if self.h:
             lib().qbh_species_destroy(self.h)
             self.h = _vp()
+
+
+def auto_rcps(zval, a, ecut):
+    """AtomSet::set_rcps's Gaussian width for valence zval, cell a (rows), cutoff ecut (Ha)."""
+    a_, ap = _da(np.asarray(a, dtype=np.float64).reshape(9))
+    out = ctypes.c_double(0.0)
+    if lib().qbh_species_auto_rcps(float(zval), ap, float(ecut), ctypes.byref(out)) != 0:
+        raise QbhError("qbh_species_auto_rcps failed")
+    return out.value
 
 
 def local_tables(vbasis, species, na, tau_ptr, omega, vion_ptr, rhops_ptr):

@@ -801,6 +801,23 @@ extern "C" int qbh_local_tables(qbh_basis vbasis, int nsp, const qbh_species* sp
   return QBH_OK;
 }
 
+// AtomSet::set_rcps (AtomSet.cc:1102-1163): from 0.3 in steps of 0.05 until the Gaussian
+// charge's reciprocal tail (zval / sqrt(pi)) erfc(rcps sqrt(ecut)) is at most 1e-7 rcps /
+// zval, capped at min_wsdist / (2 pi), then down in steps of 0.05 until the charge outside
+// the box zval erfc(min_wsdist / (sqrt(2) rcps)) is at most 1e-8
+extern "C" int qbh_species_auto_rcps(double zval, const double* a, double ecut, double* rcps) {
+  if (!a || !rcps || !(zval > 0.0) || !(ecut > 0.0)) return QBH_ERR_ARG;
+  const double ellmin = min_wsdist(a);
+  if (!(ellmin > 0.0)) return QBH_ERR_ARG;
+  double r = 0.3;
+  const double recipmax = ellmin / (1.0 * 2. * M_PI);
+  while ((zval / sqrt(M_PI)) * erfc(r * sqrt(ecut)) > 1.e-7 * r / zval) r += 0.05;
+  if (r > recipmax) r = recipmax;
+  while (zval * erfc(ellmin / (sqrt(2.) * r)) > 1.e-8) r -= 0.05;
+  *rcps = r;
+  return QBH_OK;
+}
+
 extern "C" int qbh_ewald_esr(const double* a, int nsp, const qbh_species* sps, const int* na,
                              const double* tau, double* esr) {
   if (!a || !esr || nsp < 0 || (nsp > 0 && (!sps || !na || !tau))) return QBH_ERR_ARG;

@@ -405,27 +405,67 @@ extern "C" int qbh_batch_launch(qbh_batch bt) {
 }
 
 // ------------------------------------------------------------------------------------
-// Stream bandwidth of the device: grid-stride read-only, write-only and copy kernels over
-// two buffers of `bytes`, 32-byte elements, best of `reps` launches each.  The copy rate
-// counts read plus written bytes.  What a plain streaming pass reaches on this GPU is the
-// practical ceiling against which the transform passes' HBM rates are read.
+// Stream bandwidth of the device: read-only, write-only and copy kernels over two buffers
+// of `bytes`, best over launch shapes of 1024, 4096 and 16384 workgroups of 256 threads
+// with 1, 4 or 8 independent 16-byte accesses in flight per thread per iteration, and of
+// `reps` launches each.  The copy rate counts read plus written bytes.  What a plain
+// streaming pass reaches on this GPU is the practical ceiling against which the transform
+// passes' HBM rates are read.
 namespace qbh {
+template <int U>
 __global__ void __launch_bounds__(256)
-k_bw_read(const double4* __restrict__ a, long n, double* __restrict__ sink) {
+k_bw_read(const double2* __restrict__ a, long n, double* __restrict__ sink) {
   double s = 0.0;
-  for (long i = blockIdx.x * 256L + threadIdx.x; i < n; i += (long)gridDim.x * 256) {
-    const double4 v = a[i];
-    s += (v.x + v.y) + (v.z + v.w);
+  const long stride = (long)gridDim.x * 256 * U;
+  for (long i0 = blockIdx.x * 256L * U + threadIdx.x; i0 < n; i0 += stride) {
+    double2 v[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const long i = i0 + u * 256L;
+      v[u] = i < n ? a[i] : make_double2(0.0, 0.0);
+    }
+#pragma unroll
+    for (int u = 0; u < U; ++u) s += v[u].x + v[u].y;
   }
   if (s == -1.0) sink[0] = s;  // the buffer holds non-negative values: never stored
 }
-__global__ void __launch_bounds__(256) k_bw_write(double4* __restrict__ a, long n) {
-  for (long i = blockIdx.x * 256L + threadIdx.x; i < n; i += (long)gridDim.x * 256)
-    a[i] = make_double4(1.0, 2.0, 3.0, 4.0);
+template <int U>
+__global__ void __launch_bounds__(256) k_bw_write(double2* __restrict__ a, long n) {
+  const long stride = (long)gridDim.x * 256 * U;
+  for (long i0 = blockIdx.x * 256L * U + threadIdx.x; i0 < n; i0 += stride) {
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const long i = i0 + u * 256L;
+      if (i < n) a[i] = make_double2(1.0, 2.0);
+    }
+  }
 }
+template <int U>
 __global__ void __launch_bounds__(256)
-k_bw_copy(const double4* __restrict__ a, double4* __restrict__ b, long n) {
-  for (long i = blockIdx.x * 256L + threadIdx.x; i < n; i += (long)gridDim.x * 256) b[i] = a[i];
+k_bw_copy(const double2* __restrict__ a, double2* __restrict__ b, long n) {
+  const long stride = (long)gridDim.x * 256 * U;
+  for (long i0 = blockIdx.x * 256L * U + threadIdx.x; i0 < n; i0 += stride) {
+    double2 v[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const long i = i0 + u * 256L;
+      v[u] = i < n ? a[i] : make_double2(0.0, 0.0);
+    }
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const long i = i0 + u * 256L;
+      if (i < n) b[i] = v[u];
+    }
+  }
+}
+
+template <int U>
+static void bw_launch(int k, unsigned grid, hipStream_t st, void* a, void* b, long n, void* sink) {
+  if (k == 0) k_bw_read<U><<<grid, 256, 0, st>>>(static_cast<const double2*>(a), n,
+                                                 static_cast<double*>(sink));
+  if (k == 1) k_bw_write<U><<<grid, 256, 0, st>>>(static_cast<double2*>(b), n);
+  if (k == 2) k_bw_copy<U><<<grid, 256, 0, st>>>(static_cast<const double2*>(a),
+                                                 static_cast<double2*>(b), n);
 }
 }  // namespace qbh
 
@@ -435,8 +475,8 @@ extern "C" int qbh_ctx_stream_bandwidth(qbh_ctx ctx, double bytes, int reps, dou
     return QBH_ERR_ARG;
   Ctx* c = ctx_of(ctx);
   hipSetDevice(c->device);
-  const long n = (long)(bytes / sizeof(double4));
-  const size_t nb = (size_t)n * sizeof(double4);
+  const long n = (long)(bytes / sizeof(double2));
+  const size_t nb = (size_t)n * sizeof(double2);
   void *a = nullptr, *b = nullptr, *sink = nullptr;
   hipEvent_t e0 = nullptr, e1 = nullptr;
   int r = QBH_OK;
@@ -447,26 +487,27 @@ extern "C" int qbh_ctx_stream_bandwidth(qbh_ctx ctx, double bytes, int reps, dou
     hipEventCreate(&e0);
     hipEventCreate(&e1);
     // a holds non-negative values (read and copied), b is written
-    k_bw_write<<<2048, 256, 0, c->stream>>>(static_cast<double4*>(a), n);
+    k_bw_write<1><<<2048, 256, 0, c->stream>>>(static_cast<double2*>(a), n);
     float best[3] = {1e30f, 1e30f, 1e30f};
-    for (int rep = 0; rep <= reps && r == QBH_OK; ++rep) {  // launch 0 of each is a warm-up
-      for (int k = 0; k < 3; ++k) {
-        hipEventRecord(e0, c->stream);
-        if (k == 0) k_bw_read<<<16384, 256, 0, c->stream>>>(static_cast<const double4*>(a), n,
-                                                           static_cast<double*>(sink));
-        if (k == 1) k_bw_write<<<16384, 256, 0, c->stream>>>(static_cast<double4*>(b), n);
-        if (k == 2) k_bw_copy<<<16384, 256, 0, c->stream>>>(static_cast<const double4*>(a),
-                                                           static_cast<double4*>(b), n);
-        hipEventRecord(e1, c->stream);
-        if (hipEventSynchronize(e1) != hipSuccess) {
-          r = ufail(c, QBH_ERR_DEVICE, "bandwidth kernels failed");
-          break;
-        }
-        float t = 0.f;
-        hipEventElapsedTime(&t, e0, e1);
-        if (rep > 0 && t < best[k]) best[k] = t;
-      }
-    }
+    const unsigned grids[3] = {1024, 4096, 16384};
+    const int unrolls[3] = {1, 4, 8};
+    for (int k = 0; k < 3 && r == QBH_OK; ++k)
+      for (unsigned grid : grids)
+        for (int u : unrolls)
+          for (int rep = 0; rep <= reps && r == QBH_OK; ++rep) {  // launch 0 is a warm-up
+            hipEventRecord(e0, c->stream);
+            if (u == 1) bw_launch<1>(k, grid, c->stream, a, b, n, sink);
+            if (u == 4) bw_launch<4>(k, grid, c->stream, a, b, n, sink);
+            if (u == 8) bw_launch<8>(k, grid, c->stream, a, b, n, sink);
+            hipEventRecord(e1, c->stream);
+            if (hipEventSynchronize(e1) != hipSuccess) {
+              r = ufail(c, QBH_ERR_DEVICE, "bandwidth kernels failed");
+              break;
+            }
+            float t = 0.f;
+            hipEventElapsedTime(&t, e0, e1);
+            if (rep > 0 && t < best[k]) best[k] = t;
+          }
     if (r == QBH_OK) {
       gbps[0] = nb / (best[0] * 1e-3) / 1e9;
       gbps[1] = nb / (best[1] * 1e-3) / 1e9;

@@ -80,3 +80,12 @@ def test_species_from_arrays_equals_species_from_file(pair):
     for which in (-1, 0, 1):
         assert np.array_equal(sp.eval(which, g)[0], lib_sp.eval(which, g)[0])
     sp.close()
+
+
+def test_auto_rcps_matches_oracle():
+    from oracle import species as S
+    from qball_amd import auto_rcps
+    for a, zv, ecut in [(np.diag([41.04] * 3), 4.0, 12.5), (np.diag([38.25] * 3), 3.0, 15.0),
+                        (np.diag([5.0] * 3), 4.0, 40.0),
+                        (np.array([[0.0, 2.0, 2.0], [2.0, 0.0, 2.0], [2.0, 2.0, 0.0]]), 4.0, 40.0)]:
+        assert auto_rcps(zv, a, ecut) == S.auto_rcps(zv, a, ecut)

@@ -19,6 +19,7 @@ import os
 import numpy as np
 import pytest
 from scipy.interpolate import CubicSpline
+from scipy.special import erfc
 
 from oracle import species as S
 
@@ -189,3 +190,20 @@ def test_twnl_row_order_follows_the_reference():
     rows, ls, _ = S.twnl_table(Flat(), np.array([[0.0, 0.0, 2.0]]))
     assert np.flatnonzero(np.abs(rows[ls == 2][:, 0]) > 1e-15).tolist() == [0]
     assert np.flatnonzero(np.abs(rows[ls == 3][:, 0]) > 1e-15).tolist() == [0]
+
+
+@pytest.mark.parametrize("a,zv,ecut", [(np.diag([41.04] * 3), 4.0, 12.5),
+                                        (np.diag([10.26] * 3), 4.0, 12.5),
+                                        (np.diag([6.0] * 3), 3.0, 30.0),
+                                        (np.array([[0.0, 2.0, 2.0], [2.0, 0.0, 2.0],
+                                                   [2.0, 2.0, 0.0]]), 4.0, 40.0)])
+def test_auto_rcps_is_the_first_width_meeting_both_thresholds(a, zv, ecut):
+    """AtomSet::set_rcps: the reciprocal tail test holds at the result and fails one step
+    below it (unless the cell caps the width), and the enclosed-charge test holds."""
+    r = S.auto_rcps(zv, a, ecut)
+    ell = S.min_wsdist(a)
+    tail = lambda x: (zv / math.sqrt(math.pi)) * erfc(x * math.sqrt(ecut)) - 1e-7 * x / zv
+    from scipy.special import erfc as erfc_
+    assert zv * erfc_(ell / (math.sqrt(2.0) * r)) <= 1e-8
+    if r < ell / (2.0 * math.pi) - 1e-12 and abs((r - 0.3) / 0.05 - round((r - 0.3) / 0.05)) < 1e-9:
+        assert tail(r) <= 0.0 and tail(r - 0.05) > 0.0
