@@ -150,10 +150,67 @@ def update_rhor(vmaps, rhog, omega):
     return offt.backward(vmaps, rhog).ravel().real / omega
 
 
+EVOLT = 0.036749023  # 1 eV in Hartree (Wavefunction.cc:1246)
+
+
+def fermi(e, mu, width):
+    """SlaterDet::fermi (SlaterDet.cc:1897-1915): e, mu in Hartree, width in Rydberg."""
+    e = np.asarray(e, dtype=np.float64)
+    if width == 0.0:
+        return np.where(e < mu, 1.0, np.where(e == mu, 0.5, 0.0))
+    kt = 0.5 * width
+    arg = (e - mu) / kt
+    out = 1.0 / (1.0 + np.exp(np.clip(arg, -30.0, 30.0)))
+    return np.where(arg < -30.0, 1.0, np.where(arg > 30.0, 0.0, out))
+
+
+def update_occ(eig, nel, width, maxiter=10000):
+    """Wavefunction::update_occ, one spin, one k-point of weight 1, Fermi smearing
+    (Wavefunction.cc:1214-1361, SlaterDet::update_occ :1840-1881): mu by the reference's
+    step-halving search from 0 with steps of 10 eV until |sum occ - nel| <= 1e-8;
+    occ = 2 f(eig, mu, width).  Returns (occ, mu)."""
+    mu, dmu, up = 0.0, 10.0 * EVOLT, True
+    occ = 2.0 * fermi(eig, mu, width)
+    rhosum = float(np.sum(occ))
+    niter = 0
+    while niter < maxiter and abs(rhosum - nel) > 1.0e-8:
+        niter += 1
+        if rhosum < nel:
+            if not up:
+                dmu /= 2.0
+            mu += dmu
+            up = True
+        else:
+            if up:
+                dmu /= 2.0
+            mu -= dmu
+            up = False
+        occ = 2.0 * fermi(eig, mu, width)
+        rhosum = float(np.sum(occ))
+    return occ, mu
+
+
+def entropy(occ):
+    """SlaterDet::entropy, one spin (SlaterDet.cc:1948-1962): -sum 2 (f ln f + (1-f)
+    ln(1-f)), f = occ / 2, over 0 < f < 1."""
+    f = np.asarray(occ, dtype=np.float64) / 2.0
+    m = (f > 0.0) & (f < 1.0)
+    fm = f[m]
+    return float(-np.sum(2.0 * (fm * np.log(fm) + (1.0 - fm) * np.log(1.0 - fm))))
+
+
 def scf(maps, vmaps, species, c, occ, omega, vion_g, rhops_g, nitscf, nite, ecutprec,
-        alpha=0.5, ndim=3, rc_kerker=10.0, hpsi=None):
+        alpha=0.5, ndim=3, rc_kerker=10.0, hpsi=None, smearing=0.0, nel=None):
     """Returns (c, history) with history[itscf] = list of per-ite dicts (etotal and parts,
-    PSDA theta).  ``hpsi(c)`` may replace the oracle Hψ (returns (hc, enl))."""
+    PSDA theta).  ``hpsi(c)`` may replace the oracle Hψ (returns (hc, enl)).
+
+    smearing > 0 (Rydberg, Fermi): fractional occupations as BOSampleStepper runs them
+    (BOSampleStepper.cc:1150-1185): after each cycle's iterations Hψ, the subspace
+    diagonalisation with eigenvectors (Wavefunction::diag), update_occ for nel electrons;
+    etotal carries ets = -entropy * smearing / 2 (EnergyFunctional.cc:1602-1609).  The
+    history then also holds each cycle's eigenvalues, occupations and mu (``cycles``)."""
+    occ = np.array(occ, dtype=np.float64)
+    cycles = []
     basis, vb = maps.basis, vmaps.basis
     if hpsi is None:
         def hpsi(x):
@@ -181,10 +238,18 @@ def scf(maps, vmaps, species, c, occ, omega, vion_g, rhops_g, nitscf, nite, ecut
         for ite in range(max(nite, 1)):
             hc, enl = hpsi(c)
             ekin = kinetic_energy(basis, c, occ)
-            etot = ekin + enl + eps + ehart + exc
+            ets = -entropy(occ) * smearing * 0.5 if smearing > 0.0 else 0.0
+            etot = ekin + enl + eps + ehart + exc + ets
             c, theta = psda.update(c, hc, occ)
             c = oh.gram(c, basis.real)
             steps.append(dict(etotal=etot, ekin=ekin, enl=enl, eps=eps, ehart=ehart, exc=exc,
-                              theta=theta))
+                              theta=theta, ets=ets))
         hist.append(steps)
+        if smearing > 0.0:
+            hc, _ = hpsi(c)
+            eig, c = oh.subspace_diag(c, hc, basis.real)
+            occ, mu = update_occ(eig, nel, smearing)
+            cycles.append(dict(eig=eig, occ=occ.copy(), mu=mu))
+    if smearing > 0.0:
+        return c, hist, cycles
     return c, hist

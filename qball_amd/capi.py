@@ -78,6 +78,9 @@ SIGNATURES = {
     "qbh_scf_run": (_c_int, [_vp, _vp, _vp, _c_long, _c_int, _dp, _c_double, _vp, _vp, _c_int,
                              _c_int, _c_double, _c_double, _c_int, _c_double, _c_int, _c_int,
                              _dp]),
+    "qbh_scf_run_smeared": (_c_int, [_vp, _vp, _vp, _c_long, _c_int, _dp, _c_double, _c_double,
+                                     _c_double, _vp, _vp, _c_int, _c_int, _c_double, _c_double,
+                                     _c_int, _c_double, _c_int, _c_int, _dp, _dp, _dp]),
     "qbh_ham_set_tuning": (_c_int, [_vp, ctypes.c_char_p, _c_int]),
     "qbh_ham_get_tuning": (_c_int, [_vp, ctypes.c_char_p, _ip]),
     "qbh_ham_autotune": (_c_int, [_vp, _vp, _c_long, _c_int, _dp, _vp, _c_long, _c_int, _c_int,
@@ -460,6 +463,25 @@ class Hamiltonian:
                                          float(rc_kerker), int(flags_inner), int(flags_last),
                                          hist.ctypes.data_as(_dp)), "qbh_scf_run")
         return hist
+
+    def scf_run_smeared(self, vbasis, psi_ptr, ld, nst, occ, nel, smearing, omega, vion_g_ptr,
+                        rhops_g_ptr, nitscf, nite, ecutprec, alpha=0.5, ndim=3, rc_kerker=10.0,
+                        flags_inner=QBH_ALL, flags_last=QBH_ALL):
+        """scf_run with Fermi smearing of width `smearing` (Rydberg) for nel electrons;
+        returns (history, final occupations, eigenvalues per cycle (nitscf, nst), mu per
+        cycle)."""
+        o = np.array(occ, dtype=np.float64)
+        nin = max(nite, 1)
+        hist = np.zeros((nitscf, nin, 7))
+        eig = np.zeros((nitscf, nst))
+        mu = np.zeros(nitscf)
+        self.ctx.check(lib().qbh_scf_run_smeared(
+            self.h, vbasis.h, psi_ptr, ld, nst, o.ctypes.data_as(_dp), float(nel),
+            float(smearing), float(omega), vion_g_ptr, rhops_g_ptr, int(nitscf), int(nite),
+            float(ecutprec), float(alpha), int(ndim), float(rc_kerker), int(flags_inner),
+            int(flags_last), hist.ctypes.data_as(_dp), eig.ctypes.data_as(_dp),
+            mu.ctypes.data_as(_dp)), "qbh_scf_run_smeared")
+        return hist, o, eig, mu
 
     def set_tuning(self, knob, value):
         self.ctx.check(lib().qbh_ham_set_tuning(self.h, knob.encode(), int(value)),

@@ -510,13 +510,65 @@ extern "C" int qbh_mixer_update(qbh_mixer mx, const double* x_dev, const double*
 }
 
 // ---------------------------------------------------------------------------- SCF loop
-extern "C" int qbh_scf_run(qbh_ham ham, qbh_basis vbasis, double* psi_dev, long ld, int nst,
-                           const double* occ, double omega, const double* vion_g_dev,
-                           const double* rhops_g_dev, int nitscf, int nite, double ecutprec,
-                           double mix_alpha, int mix_ndim, double rc_kerker, int flags_inner,
-                           int flags_last, double* history) {
-  if (!ham || !vbasis || !psi_dev || !occ || nst <= 0 || nitscf < 1 || !(omega > 0.0))
-    return QBH_ERR_ARG;
+// SlaterDet::fermi (SlaterDet.cc:1897-1915): e, mu in Hartree, width in Rydberg
+static double fermi(double e, double mu, double width) {
+  if (width == 0.0) return e < mu ? 1.0 : (e == mu ? 0.5 : 0.0);
+  const double kt = 0.5 * width;
+  const double arg = (e - mu) / kt;
+  if (arg < -30.0) return 1.0;
+  if (arg > 30.0) return 0.0;
+  return 1.0 / (1.0 + exp(arg));
+}
+
+// Wavefunction::update_occ, one spin and one k-point of weight 1, Fermi smearing
+// (Wavefunction.cc:1214-1361 with SlaterDet::update_occ, SlaterDet.cc:1840-1881): mu by
+// step-halving from 0 in steps of 10 eV until |sum occ - nel| <= 1e-8; occ = 2 f
+static double update_occ(const std::vector<double>& eig, double nel, double width,
+                         std::vector<double>& occ) {
+  const double eVolt = 0.036749023;
+  double mu = 0.0, dmu = 10.0 * eVolt;
+  bool up = true;
+  auto fill = [&]() {
+    double s = 0.0;
+    for (size_t n = 0; n < eig.size(); ++n) {
+      occ[n] = 2.0 * fermi(eig[n], mu, width);
+      s += occ[n];
+    }
+    return s;
+  };
+  double rhosum = fill();
+  for (int niter = 0; niter < 10000 && fabs(rhosum - nel) > 1.e-8; ++niter) {
+    if (rhosum < nel) {
+      if (!up) dmu /= 2.0;
+      mu += dmu;
+      up = true;
+    } else {
+      if (up) dmu /= 2.0;
+      mu -= dmu;
+      up = false;
+    }
+    rhosum = fill();
+  }
+  return mu;
+}
+
+// SlaterDet::entropy, one spin (SlaterDet.cc:1948-1962)
+static double entropy(const std::vector<double>& occ) {
+  double sum = 0.0;
+  for (double o : occ) {
+    const double f = o / 2.0;
+    if (f > 0.0 && f < 1.0) sum -= 2.0 * (f * log(f) + (1.0 - f) * log(1.0 - f));
+  }
+  return sum;
+}
+
+static int scf_impl(qbh_ham ham, qbh_basis vbasis, double* psi_dev, long ld, int nst,
+                    std::vector<double>& occv, double nel, double width, double omega,
+                    const double* vion_g_dev, const double* rhops_g_dev, int nitscf, int nite,
+                    double ecutprec, double mix_alpha, int mix_ndim, double rc_kerker,
+                    int flags_inner, int flags_last, double* history, double* eig_out,
+                    double* mu_out) {
+  const double* occ = occv.data();
   Ham* h = &ham->h;
   Ctx* c = h->ctx;
   hipSetDevice(c->device);
@@ -587,9 +639,11 @@ extern "C" int qbh_scf_run(qbh_ham ham, qbh_basis vbasis, double* psi_dev, long 
       if ((r = qbh_kinetic_energy(wb, psi_dev, ld, nst, occ, &ekin))) break;
       if ((r = qbh_psda_update(psda, psi_dev, ld, hc, g.ngw, occ, &th))) break;
       if ((r = qbh_gram(wb, psi_dev, ld, nst))) break;
+      // EnergyFunctional.cc:1602-1609: ets = -entropy * smearing_width / 2
+      const double ets = width > 0.0 ? -entropy(occv) * width * 0.5 : 0.0;
       if (history) {
         double* hrow = history + 7 * ((size_t)it * nin + ite);
-        hrow[0] = ekin + enl + e3[0] + e3[1] + e3[2];
+        hrow[0] = ekin + enl + e3[0] + e3[1] + e3[2] + ets;
         hrow[1] = ekin;
         hrow[2] = enl;
         hrow[3] = e3[0];
@@ -598,6 +652,18 @@ extern "C" int qbh_scf_run(qbh_ham ham, qbh_basis vbasis, double* psi_dev, long 
         hrow[6] = th;
       }
     }
+    if (r == QBH_OK && width > 0.0) {
+      // BOSampleStepper.cc:1150-1185: energy with Hψ, Wavefunction::diag with
+      // eigenvectors, update_occ
+      std::vector<double> eig(nst);
+      if ((r = hpsi_impl(h, psi_dev, ld, nst, occ, hc, g.ngw, whole_h_flags(flags_last),
+                         nullptr)))
+        break;
+      if ((r = qbh_subspace_diag(wb, psi_dev, ld, hc, g.ngw, nst, eig.data(), 1))) break;
+      const double mu = update_occ(eig, nel, width, occv);
+      if (eig_out) std::copy(eig.begin(), eig.end(), eig_out + (size_t)it * nst);
+      if (mu_out) mu_out[it] = mu;
+    }
   }
   hipStreamSynchronize(c->stream);
   if (psda) qbh_psda_destroy(psda);
@@ -605,4 +671,35 @@ extern "C" int qbh_scf_run(qbh_ham ham, qbh_basis vbasis, double* psi_dev, long 
   hipFree(mem);
   if (r) return r;
   return scheck(c, "scf");
+}
+
+extern "C" int qbh_scf_run(qbh_ham ham, qbh_basis vbasis, double* psi_dev, long ld, int nst,
+                           const double* occ, double omega, const double* vion_g_dev,
+                           const double* rhops_g_dev, int nitscf, int nite, double ecutprec,
+                           double mix_alpha, int mix_ndim, double rc_kerker, int flags_inner,
+                           int flags_last, double* history) {
+  if (!ham || !vbasis || !psi_dev || !occ || nst <= 0 || nitscf < 1 || !(omega > 0.0))
+    return QBH_ERR_ARG;
+  std::vector<double> o(occ, occ + nst);
+  return scf_impl(ham, vbasis, psi_dev, ld, nst, o, 0.0, 0.0, omega, vion_g_dev, rhops_g_dev,
+                  nitscf, nite, ecutprec, mix_alpha, mix_ndim, rc_kerker, flags_inner, flags_last,
+                  history, nullptr, nullptr);
+}
+
+extern "C" int qbh_scf_run_smeared(qbh_ham ham, qbh_basis vbasis, double* psi_dev, long ld,
+                                   int nst, double* occ, double nel, double smearing,
+                                   double omega, const double* vion_g_dev,
+                                   const double* rhops_g_dev, int nitscf, int nite,
+                                   double ecutprec, double mix_alpha, int mix_ndim,
+                                   double rc_kerker, int flags_inner, int flags_last,
+                                   double* history, double* eig, double* mu) {
+  if (!ham || !vbasis || !psi_dev || !occ || nst <= 0 || nitscf < 1 || !(omega > 0.0) ||
+      !(smearing > 0.0) || !(nel > 0.0) || nel > 2.0 * nst)
+    return QBH_ERR_ARG;
+  std::vector<double> o(occ, occ + nst);
+  const int r = scf_impl(ham, vbasis, psi_dev, ld, nst, o, nel, smearing, omega, vion_g_dev,
+                         rhops_g_dev, nitscf, nite, ecutprec, mix_alpha, mix_ndim, rc_kerker,
+                         flags_inner, flags_last, history, eig, mu);
+  std::copy(o.begin(), o.end(), occ);
+  return r;
 }

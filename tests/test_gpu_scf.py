@@ -179,3 +179,94 @@ def test_scf_flags_without_terms_mean_the_whole_hamiltonian(gpu_ctx):
     assert np.max(np.abs(hists[0] - hists[1])) <= 1e-12 * np.max(np.abs(hists[0]))
     assert np.all(np.abs(hists[0][:, :, 1]) > 0)        # kinetic energy is there
     _close(s)
+
+
+def test_scf_on_the_pseudopotential_matches_oracle(gpu_ctx):
+    """Si8 Γ on the Si HSCV LDA species (rcps by the AtomSet rule): projector tables, v_ion
+    and rho_ps made by the library, three SCF cycles; etotal with the ion-ion terms esr -
+    eself against the oracle's loop on its own species tables (1e-9 Ha)."""
+    import os
+    import torch
+    from oracle import hpsi as oh
+    from oracle import species as S
+    from oracle.basis import Basis as OB, FFTMaps, UnitCell
+    from oracle.scf import scf
+    from qball_amd import Basis, Species, auto_rcps, capi, ewald_esr, local_tables
+    from qball_amd.workload import make_system
+    xml = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "Si_HSCV_LDA-1.0.xml")
+    sysm = make_system(ncell=1, ecut_ry=25.0, nbands=16, seed=46)
+    ob, maps, _ = oracle_for(sysm)
+    vob = OB(UnitCell(sysm.a), 4.0 * sysm.ecut)
+    vmaps = FFTMaps(vob, *sysm.np)
+    rc = auto_rcps(4.0, sysm.a, sysm.ecut)
+    assert rc == S.auto_rcps(4.0, sysm.a, sysm.ecut)
+    sp = Species(xml, rcps=rc)
+    ora = S.Species(S.read_species_xml(xml), rc)
+    basis = make_gpu_basis(gpu_ctx, sysm)
+    ham = make_gpu_ham(gpu_ctx, basis, sysm, with_proj=False)
+    ham.set_species([sp], [sysm.na], sysm.tau, sysm.omega)
+    r = capi.rods_build(sysm.a, 4.0 * sysm.ecut, True)
+    vb = Basis(gpu_ctx, True, r["rod_h"], r["rod_k"], r["rod_lmin"], r["rod_size"], sysm.np, sysm.b)
+    assert vb.ngw == vob.size
+    tau_d = torch.from_numpy(np.ascontiguousarray(sysm.tau).ravel().copy()).cuda()
+    vg = torch.zeros(2 * vb.ngw, dtype=torch.float64, device="cuda")
+    rg = torch.zeros_like(vg)
+    local_tables(vb, [sp], [sysm.na], tau_d.data_ptr(), sysm.omega, vg.data_ptr(), rg.data_ptr())
+    esr = ewald_esr(sysm.a, [sp], [sysm.na], sysm.tau)
+    esr_ref = S.esr(sysm.a, [ora.zval], [rc], [sysm.tau])
+    assert abs(esr - esr_ref) <= 1e-12 * max(1.0, abs(esr_ref))
+    eself = sysm.na * sp.eself
+    rows, ls, wt = S.twnl_table(ora, ob.kpgx)
+    vion, rhops = S.local_tables([ora], [sysm.tau], vob.kpgx, sysm.omega)
+    psi = oh.gram(sysm.psi, True)
+    occ = np.full(sysm.nst, 2.0)
+    nitscf, nite = 3, 2
+    cd = to_dev(psi)
+    torch.cuda.synchronize()
+    hist = ham.scf_run(vb, cd.data_ptr(), basis.ngw, sysm.nst, occ, sysm.omega, vg.data_ptr(),
+                       rg.data_ptr(), nitscf, nite, sysm.ecut)
+    gpu_ctx.sync()
+    _, h_ref = scf(maps, vmaps, [oh.Projectors(ls, wt, rows, sysm.tau)], psi, occ, sysm.omega,
+                   vion, rhops, nitscf, nite, sysm.ecut)
+    eself_ref = sysm.na * ora.zval ** 2 / (np.sqrt(2.0 * np.pi) * rc)
+    for i in range(nitscf):
+        for j in range(nite):
+            got = hist[i, j, 0] + esr - eself
+            want = h_ref[i][j]["etotal"] + esr_ref - eself_ref
+            assert abs(got - want) <= 1e-9, (i, j, got, want)
+    print("etotal per atom after", nitscf, "cycles:", (hist[-1, -1, 0] + esr - eself) / sysm.na)
+    ham.close()
+    basis.close()
+    vb.close()
+    sp.close()
+
+
+def test_smeared_scf_matches_oracle(gpu_ctx):
+    """Fermi smearing (qbh_scf_run_smeared, width 0.02 Ry): Si8 Γ with 20 bands for 30
+    electrons, three cycles; per iteration etotal (with ets) and per cycle the subspace
+    eigenvalues, the occupations and mu against the oracle's loop (1e-9)."""
+    from oracle.scf import scf
+    s = _setup(gpu_ctx, "g", 20, 47)
+    sysm = s["sysm"]
+    nel, width = 30.0, 0.02
+    occ0 = np.full(sysm.nst, nel / sysm.nst)
+    nitscf, nite = 3, 2
+    cd = to_dev(s["psi"])
+    hist, occ, eig, mu = s["ham"].scf_run_smeared(
+        s["vb"], cd.data_ptr(), s["basis"].ngw, sysm.nst, occ0, nel, width, sysm.omega,
+        s["vd"].data_ptr(), s["pd"].data_ptr(), nitscf, nite, sysm.ecut)
+    gpu_ctx.sync()
+    c_ref, h_ref, cyc = scf(s["maps"], s["vmaps"], [s["sp"]], s["psi"], occ0, sysm.omega,
+                            s["vion"], s["rhops"], nitscf, nite, sysm.ecut, smearing=width,
+                            nel=nel)
+    for i in range(nitscf):
+        for j in range(nite):
+            r = h_ref[i][j]
+            assert abs(hist[i, j, 0] - r["etotal"]) <= 1e-9, (i, j)
+        assert np.max(np.abs(eig[i] - cyc[i]["eig"])) <= 1e-9, i
+        assert abs(mu[i] - cyc[i]["mu"]) <= 1e-9, i
+    assert np.max(np.abs(occ - cyc[-1]["occ"])) <= 1e-9
+    assert abs(occ.sum() - nel) <= 1e-8
+    assert np.any((occ > 1e-3) & (occ < 2.0 - 1e-3))   # fractional occupations occur
+    assert any(abs(h_ref[i][j]["ets"]) > 0 for i in range(1, nitscf) for j in range(nite))
+    _close(s)

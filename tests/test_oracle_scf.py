@@ -129,3 +129,37 @@ def test_mixing_weights_half_points():
     assert wk[0] == 0.0 and wls[0] == 1.0 and abs(wk[3] - 1.0) < 1e-5
     wk0, _ = mixing_weights(vb, rc_kerker=0.0)
     assert np.all(wk0 == 1.0)
+
+
+def test_fermi_occupations_minimise_the_free_energy():
+    """Wavefunction::update_occ / SlaterDet::fermi / SlaterDet::entropy: the occupations hold
+    nel electrons to the reference's 1e-8, mu agrees with a root of N(mu) = nel, and with
+    kT = width / 2 (width in Rydberg) they minimise F = sum occ e - kT S over occupations of
+    the same total, S the entropy the reference's ets uses (ets = -S width / 2)."""
+    from scipy.optimize import brentq
+    from oracle import scf as O
+    rng = np.random.default_rng(4)
+    eig = np.sort(rng.uniform(-0.6, 0.4, 30))
+    nel, width = 33.7, 0.02
+    occ, mu = O.update_occ(eig, nel, width)
+    assert abs(occ.sum() - nel) <= 1e-8
+    mu_root = brentq(lambda m: np.sum(2.0 / (1.0 + np.exp((eig - m) / (0.5 * width)))) - nel,
+                     -2.0, 2.0, xtol=1e-15)
+    dn_dmu = np.sum(occ * (2.0 - occ) / 2.0) / (0.5 * width)
+    assert abs(mu - mu_root) <= 2e-8 / dn_dmu
+    kt = 0.5 * width
+    F = lambda o: float(np.sum(o * eig)) - kt * O.entropy(o)
+    f0 = F(occ)
+    wgt = occ * (2.0 - occ)     # perturbations that keep every occupation inside (0, 2)
+    for _ in range(20):
+        d = 0.05 * rng.standard_normal(eig.size) * wgt
+        d -= wgt * d.sum() / wgt.sum()
+        o = occ + d
+        assert np.all((o >= 0.0) & (o <= 2.0)) and abs(o.sum() - occ.sum()) < 1e-12
+        assert F(o) > f0
+    # limits: a full or empty state has no entropy, f = 1/2 gives 2 ln 2
+    assert O.entropy(np.array([2.0, 0.0])) == 0.0
+    assert abs(O.entropy(np.array([1.0])) - 2.0 * np.log(2.0)) <= 1e-15
+    # a gapped spectrum at vanishing width fills the lowest nel / 2 states
+    occ0, _ = O.update_occ(np.array([-1.0, -0.5, 0.5, 1.0]), 4.0, 1e-6)
+    assert np.allclose(occ0, [2.0, 2.0, 0.0, 0.0])
