@@ -293,6 +293,8 @@ struct qbh_td_s {
   bool nl_adj = false;
   int nrows = 0;
   double *tw = nullptr, *dtw = nullptr, *anl4 = nullptr, *fz4 = nullptr, *fu4 = nullptr;
+  bool tab_ok = false;  // anl4 holds the tables at tab_A
+  double tab_A[3] = {0.0, 0.0, 0.0};
   size_t blk() const { return 2 * (size_t)ngw * nst; }
   size_t seg_step() const { return (3 * (size_t)order + 1) * blk() + 4 * (size_t)N; }
 };
@@ -380,8 +382,10 @@ static int cross(qbh_td_s* t, const double* zk, const double* u, const double* A
   const int P = h->P;
   const bool nl = t->nl_adj && (whole_h_flags(t->flags) & QBH_NONLOCAL);
   rocblas_handle bh = nullptr;
-  if (nl) {
-    // tables at this exponential's A: twnl rows, their gradient, anl and danl_d
+  if (nl && !(t->tab_ok && A[0] == t->tab_A[0] && A[1] == t->tab_A[1] && A[2] == t->tab_A[2])) {
+    // tables at this exponential's A: twnl rows, their gradient, anl and danl_d (kept for
+    // the next exponential under the same A)
+    t->tab_ok = false;
     const long rs = (long)t->nrows * t->ngw;
     long row0 = 0;
     for (qbh_species sp : h->sps) {
@@ -393,14 +397,19 @@ static int cross(qbh_td_s* t, const double* zk, const double* u, const double* A
     if (int r = build_anl_c64(h, t->tw, A, t->anl4)) return r;
     for (int d = 0; d < 3; ++d)
       if (int r = build_anl_c64(h, t->dtw + d * rs, A, t->anl4 + (1 + d) * cs)) return r;
+    t->tab_ok = true;
+    std::copy(A, A + 3, t->tab_A);
+  }
+  if (nl) {
     void* hv = nullptr;
     if (int r = blas_handle(cx, &hv)) return tfail(cx, r, "rocBLAS handle creation failed");
     bh = static_cast<rocblas_handle>(hv);
   }
+  // F = [anl | danl_x | danl_y | danl_z]^H x for the order blocks of x at once
   auto products = [&](const double* x, double* out) {
     const rocblas_double_complex one(1.0, 0.0), zero(0.0, 0.0);
     return rocblas_zgemm(bh, rocblas_operation_conjugate_transpose, rocblas_operation_none,
-                         4 * P, t->nst, t->ngw, &one,
+                         4 * P, o * t->nst, t->ngw, &one,
                          reinterpret_cast<const rocblas_double_complex*>(t->anl4), t->ngw,
                          reinterpret_cast<const rocblas_double_complex*>(x), t->ngw, &zero,
                          reinterpret_cast<rocblas_double_complex*>(out), 4 * P);
@@ -408,6 +417,7 @@ static int cross(qbh_td_s* t, const double* zk, const double* u, const double* A
   const int nb = t->nb_cross;
   double* fz = t->grids;
   double* fU = t->grids + 2 * (size_t)nb * t->N;
+  // U_k = sum_m alpha_{k+m+1} u_m for every k (order blocks, like zk)
   for (int k = 0; k < o; ++k) {
     Combo cb{};
     cb.cnt = o - k;
@@ -416,22 +426,26 @@ static int cross(qbh_td_s* t, const double* zk, const double* u, const double* A
       cb.ar[m] = a[k + m + 1].real();
       cb.ai[m] = a[k + m + 1].imag();
     }
-    k_combo<<<nb256(nc), 256, 0, cx->stream>>>(nc, cb, t->ubuf);
-    const double* z = zk + k * B;
-    k_crossk_part<<<NBLK, 256, 0, cx->stream>>>(t->ngw, t->nst, t->h->b->kpgx, A[0], A[1], A[2],
-                                               z, t->ubuf, t->part);
+    k_combo<<<nb256(nc), 256, 0, cx->stream>>>(nc, cb, t->ubuf + k * B);
+  }
+  if (nl) {
+    if (products(zk, t->fz4) != rocblas_status_success ||
+        products(t->ubuf, t->fu4) != rocblas_status_success)
+      return tfail(cx, QBH_ERR_BLAS, "td projector products failed");
+    k_nl_dA_part<<<NBLK, 256, 0, cx->stream>>>(P, o * t->nst, h->col_fac, t->fz4, t->fu4,
+                                               t->part);
     k_final<<<1, 256, 0, cx->stream>>>(NBLK, 3, t->part, -2.0, 1, gA3);
-    if (nl) {
-      if (products(z, t->fz4) != rocblas_status_success ||
-          products(t->ubuf, t->fu4) != rocblas_status_success)
-        return tfail(cx, QBH_ERR_BLAS, "td projector products failed");
-      k_nl_dA_part<<<NBLK, 256, 0, cx->stream>>>(P, t->nst, h->col_fac, t->fz4, t->fu4, t->part);
-      k_final<<<1, 256, 0, cx->stream>>>(NBLK, 3, t->part, -2.0, 1, gA3);
-    }
+  }
+  for (int k = 0; k < o; ++k) {
+    const double* z = zk + k * B;
+    const double* U = t->ubuf + k * B;
+    k_crossk_part<<<NBLK, 256, 0, cx->stream>>>(t->ngw, t->nst, t->h->b->kpgx, A[0], A[1], A[2],
+                                               z, U, t->part);
+    k_final<<<1, 256, 0, cx->stream>>>(NBLK, 3, t->part, -2.0, 1, gA3);
     for (int b0 = 0; b0 < t->nst; b0 += nb) {
       const int m = std::min(nb, t->nst - b0);
       if (int r = qbh_fft_backward(hb, z + 2 * (size_t)b0 * t->ngw, t->ngw, m, 0, fz, 0)) return r;
-      if (int r = qbh_fft_backward(hb, t->ubuf + 2 * (size_t)b0 * t->ngw, t->ngw, m, 0, fU, 0))
+      if (int r = qbh_fft_backward(hb, U + 2 * (size_t)b0 * t->ngw, t->ngw, m, 0, fU, 0))
         return r;
       k_cross_acc<<<nb256(t->N), 256, 0, cx->stream>>>(t->N, m, 2.0 / (double)t->N, fz, fU, mu);
     }
@@ -482,7 +496,8 @@ static int ensure_memory(qbh_td_s* t) {
   auto add = [&](size_t d) { need += ((d * sizeof(double) + 255) / 256) * 256; };
   const size_t sizes[] = {(size_t)t->nst, 2 * (size_t)NBLK * 3, 2 * (size_t)(t->nsteps + 1),
                           3 * (size_t)(t->nsteps + 1), B, B, B, B, B, 2 * (size_t)t->ngv,
-                          nck * B, t->ckpt * t->seg_step(), B, B, B, B, t->order * B, B, N, N, N,
+                          nck * B, t->ckpt * t->seg_step(), B, B, B, B, t->order * B,
+                          t->order * B, N, N, N,
                           2 * N, 2 * (size_t)t->ngv, 4 * (size_t)t->nb_cross * N, N, N, N, N};
   for (size_t s : sizes) add(s);
   Ham* h = t->h;
@@ -491,7 +506,9 @@ static int ensure_memory(qbh_td_s* t) {
   for (qbh_species sp : h->sps) t->nrows += species_nproj(sp);
   const size_t rows_d = (size_t)t->nrows * t->ngw;
   const size_t tabs[] = {rows_d, 3 * rows_d, 8 * (size_t)t->ngw * h->P,
-                         8 * (size_t)h->P * t->nst, 8 * (size_t)h->P * t->nst};
+                         8 * (size_t)h->P * t->nst * t->order,
+                         8 * (size_t)h->P * t->nst * t->order};
+  t->tab_ok = false;
   if (t->nl_adj)
     for (size_t s : tabs) add(s);
   if (need > t->mem_bytes) {
@@ -523,7 +540,7 @@ static int ensure_memory(qbh_td_s* t) {
   t->lam_t = carve(p, B);
   t->tmp = carve(p, B);
   t->zk = carve(p, t->order * B);
-  t->ubuf = carve(p, B);
+  t->ubuf = carve(p, t->order * B);
   t->mu = carve(p, N);
   t->nu = carve(p, N);
   t->gv = carve(p, N);
