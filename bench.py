@@ -57,6 +57,8 @@ def parse():
     p.add_argument("--smeared-cycles", type=int, default=8,
                    help="Fermi-smeared SCF cycles of the Al500 section")
     p.add_argument("--scf-nite", type=int, default=3)
+    p.add_argument("--scf-diag", action="store_true",
+                   help="subspace diagonalisation after every SCF cycle (wf_diag T)")
     p.add_argument("--batch-streams", type=int, default=4,
                    help="contexts (HIP streams) the 12 batch cells are spread over")
     p.add_argument("--no-tune", action="store_true",
@@ -415,9 +417,14 @@ def scf_bench(args, ctx, basis, ham, sysm, species, psi0):
         psi.upload(start)
         synchronize()
         t0 = time.perf_counter()
-        hist = ham.scf_run(vb, psi.ptr, ngw, nst, occ, sysm.omega, vion.ptr, rhops.ptr,
-                           args.scf_cycles, args.scf_nite, sysm.ecut, flags_inner=fi,
-                           flags_last=QBH_ALL)
+        if args.scf_diag:
+            hist = ham.scf_run_smeared(vb, psi.ptr, ngw, nst, occ, 2.0 * nst, 0.0, sysm.omega,
+                                       vion.ptr, rhops.ptr, args.scf_cycles, args.scf_nite,
+                                       sysm.ecut, flags_inner=fi, flags_last=QBH_ALL)[0]
+        else:
+            hist = ham.scf_run(vb, psi.ptr, ngw, nst, occ, sysm.omega, vion.ptr, rhops.ptr,
+                               args.scf_cycles, args.scf_nite, sysm.ecut, flags_inner=fi,
+                               flags_last=QBH_ALL)
         synchronize()
         t1 = time.perf_counter()
         rho.zero()
@@ -435,7 +442,8 @@ def scf_bench(args, ctx, basis, ham, sysm, species, psi0):
     a, b = runs["fp64"], runs["precision"]
     out = {"workload": f"Si{sysm.na} Γ, Si HSCV LDA (rcps {species.rcps:.2f}), {ngw} PWs, "
                        f"{nst} bands, grid {sysm.np[0]}^3, {args.scf_cycles} SCF cycles x "
-                       f"{args.scf_nite} PSDA iterations, Anderson ndim 3, Kerker rc 10",
+                       f"{args.scf_nite} PSDA iterations, Anderson ndim 3, Kerker rc 10"
+                       + (", subspace diagonalisation each cycle" if args.scf_diag else ""),
            "esr_Ha": esr, "eself_Ha": eself,
            "dE_per_atom_Ha": abs(b["etotal_Ha"] - a["etotal_Ha"]) / sysm.na,
            "density_L1_electrons": float(np.abs(b["rho"] - a["rho"]).sum() * sysm.omega / N),

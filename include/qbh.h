@@ -323,11 +323,14 @@ int qbh_scf_run(qbh_ham ham, qbh_basis vbasis, double* psi_dev, long ld, int nst
                 const double* rhops_g_dev, int nitscf, int nite, double ecutprec,
                 double mix_alpha, int mix_ndim, double rc_kerker, int flags_inner,
                 int flags_last, double* history);
-/* qbh_scf_run with fractional occupations (Fermi smearing of width `smearing`, Rydberg, as
- * the reference's smearing_width; BOSampleStepper.cc:1150-1185): after each cycle's
- * iterations Hψ (flags_last), the subspace diagonalisation with eigenvectors
- * (Wavefunction::diag) and Wavefunction::update_occ (Wavefunction.cc:1214-1361: mu by
- * step-halving from 0 in 10 eV steps until |sum occ - nel| <= 1e-8, occ = 2 f(eig, mu)).
+/* qbh_scf_run with the subspace diagonalisation each cycle and, for smearing > 0,
+ * fractional occupations (Fermi smearing of width `smearing`, Rydberg, as the reference's
+ * smearing_width; BOSampleStepper.cc:1150-1185): after each cycle's iterations Hψ
+ * (flags_last), the subspace diagonalisation with eigenvectors (Wavefunction::diag) and
+ * Wavefunction::update_occ (Wavefunction.cc:1214-1361: mu by step-halving from 0 in 10 eV
+ * steps until |sum occ - nel| <= 1e-8, occ = 2 f(eig, mu)).  smearing = 0: occupations
+ * stay as given (the reference's wf_diag = T, BOSampleStepper.cc:197), nel is not read,
+ * mu receives 0.
  * occ (host, nst) holds the starting occupations and receives the final ones; etotal in
  * history includes ets = -entropy * smearing / 2 (EnergyFunctional.cc:1602-1609); eig
  * (nullable, nitscf x nst) and mu (nullable, nitscf) receive each cycle's eigenvalues and

@@ -200,7 +200,7 @@ def entropy(occ):
 
 
 def scf(maps, vmaps, species, c, occ, omega, vion_g, rhops_g, nitscf, nite, ecutprec,
-        alpha=0.5, ndim=3, rc_kerker=10.0, hpsi=None, smearing=0.0, nel=None):
+        alpha=0.5, ndim=3, rc_kerker=10.0, hpsi=None, smearing=0.0, nel=None, diag=False):
     """Returns (c, history) with history[itscf] = list of per-ite dicts (etotal and parts,
     PSDA theta).  ``hpsi(c)`` may replace the oracle Hψ (returns (hc, enl)).
 
@@ -208,7 +208,9 @@ def scf(maps, vmaps, species, c, occ, omega, vion_g, rhops_g, nitscf, nite, ecut
     (BOSampleStepper.cc:1150-1185): after each cycle's iterations Hψ, the subspace
     diagonalisation with eigenvectors (Wavefunction::diag), update_occ for nel electrons;
     etotal carries ets = -entropy * smearing / 2 (EnergyFunctional.cc:1602-1609).  The
-    history then also holds each cycle's eigenvalues, occupations and mu (``cycles``)."""
+    history then also holds each cycle's eigenvalues, occupations and mu (``cycles``).
+    diag (wf_diag = T, BOSampleStepper.cc:197, 1150-1163): the same Hψ and subspace
+    diagonalisation with eigenvectors after each cycle, occupations unchanged."""
     occ = np.array(occ, dtype=np.float64)
     cycles = []
     basis, vb = maps.basis, vmaps.basis
@@ -245,11 +247,13 @@ def scf(maps, vmaps, species, c, occ, omega, vion_g, rhops_g, nitscf, nite, ecut
             steps.append(dict(etotal=etot, ekin=ekin, enl=enl, eps=eps, ehart=ehart, exc=exc,
                               theta=theta, ets=ets))
         hist.append(steps)
-        if smearing > 0.0:
+        if smearing > 0.0 or diag:
             hc, _ = hpsi(c)
             eig, c = oh.subspace_diag(c, hc, basis.real)
-            occ, mu = update_occ(eig, nel, smearing)
+            mu = None
+            if smearing > 0.0:
+                occ, mu = update_occ(eig, nel, smearing)
             cycles.append(dict(eig=eig, occ=occ.copy(), mu=mu))
-    if smearing > 0.0:
+    if smearing > 0.0 or diag:
         return c, hist, cycles
     return c, hist

@@ -563,7 +563,7 @@ static double entropy(const std::vector<double>& occ) {
 }
 
 static int scf_impl(qbh_ham ham, qbh_basis vbasis, double* psi_dev, long ld, int nst,
-                    std::vector<double>& occv, double nel, double width, double omega,
+                    std::vector<double>& occv, double nel, double width, bool diag, double omega,
                     const double* vion_g_dev, const double* rhops_g_dev, int nitscf, int nite,
                     double ecutprec, double mix_alpha, int mix_ndim, double rc_kerker,
                     int flags_inner, int flags_last, double* history, double* eig_out,
@@ -652,15 +652,15 @@ static int scf_impl(qbh_ham ham, qbh_basis vbasis, double* psi_dev, long ld, int
         hrow[6] = th;
       }
     }
-    if (r == QBH_OK && width > 0.0) {
+    if (r == QBH_OK && (width > 0.0 || diag)) {
       // BOSampleStepper.cc:1150-1185: energy with Hψ, Wavefunction::diag with
-      // eigenvectors, update_occ
+      // eigenvectors, then (fractional occupations) update_occ
       std::vector<double> eig(nst);
       if ((r = hpsi_impl(h, psi_dev, ld, nst, occ, hc, g.ngw, whole_h_flags(flags_last),
                          nullptr)))
         break;
       if ((r = qbh_subspace_diag(wb, psi_dev, ld, hc, g.ngw, nst, eig.data(), 1))) break;
-      const double mu = update_occ(eig, nel, width, occv);
+      const double mu = width > 0.0 ? update_occ(eig, nel, width, occv) : 0.0;
       if (eig_out) std::copy(eig.begin(), eig.end(), eig_out + (size_t)it * nst);
       if (mu_out) mu_out[it] = mu;
     }
@@ -681,9 +681,9 @@ extern "C" int qbh_scf_run(qbh_ham ham, qbh_basis vbasis, double* psi_dev, long 
   if (!ham || !vbasis || !psi_dev || !occ || nst <= 0 || nitscf < 1 || !(omega > 0.0))
     return QBH_ERR_ARG;
   std::vector<double> o(occ, occ + nst);
-  return scf_impl(ham, vbasis, psi_dev, ld, nst, o, 0.0, 0.0, omega, vion_g_dev, rhops_g_dev,
-                  nitscf, nite, ecutprec, mix_alpha, mix_ndim, rc_kerker, flags_inner, flags_last,
-                  history, nullptr, nullptr);
+  return scf_impl(ham, vbasis, psi_dev, ld, nst, o, 0.0, 0.0, false, omega, vion_g_dev,
+                  rhops_g_dev, nitscf, nite, ecutprec, mix_alpha, mix_ndim, rc_kerker,
+                  flags_inner, flags_last, history, nullptr, nullptr);
 }
 
 extern "C" int qbh_scf_run_smeared(qbh_ham ham, qbh_basis vbasis, double* psi_dev, long ld,
@@ -694,10 +694,11 @@ extern "C" int qbh_scf_run_smeared(qbh_ham ham, qbh_basis vbasis, double* psi_de
                                    double rc_kerker, int flags_inner, int flags_last,
                                    double* history, double* eig, double* mu) {
   if (!ham || !vbasis || !psi_dev || !occ || nst <= 0 || nitscf < 1 || !(omega > 0.0) ||
-      !(smearing > 0.0) || !(nel > 0.0) || nel > 2.0 * nst)
+      !(smearing >= 0.0) || (smearing > 0.0 && (!(nel > 0.0) || nel > 2.0 * nst)))
     return QBH_ERR_ARG;
   std::vector<double> o(occ, occ + nst);
-  const int r = scf_impl(ham, vbasis, psi_dev, ld, nst, o, nel, smearing, omega, vion_g_dev,
+  const int r = scf_impl(ham, vbasis, psi_dev, ld, nst, o, nel, smearing, true, omega,
+                         vion_g_dev,
                          rhops_g_dev, nitscf, nite, ecutprec, mix_alpha, mix_ndim, rc_kerker,
                          flags_inner, flags_last, history, eig, mu);
   std::copy(o.begin(), o.end(), occ);

@@ -270,3 +270,27 @@ def test_smeared_scf_matches_oracle(gpu_ctx):
     assert np.any((occ > 1e-3) & (occ < 2.0 - 1e-3))   # fractional occupations occur
     assert any(abs(h_ref[i][j]["ets"]) > 0 for i in range(1, nitscf) for j in range(nite))
     _close(s)
+
+
+def test_scf_with_subspace_diagonalisation_matches_oracle(gpu_ctx):
+    """qbh_scf_run_smeared with smearing 0 (the reference's wf_diag = T): fixed occupations,
+    Hψ and the subspace diagonalisation with eigenvectors after each cycle; etotal per
+    iteration and the eigenvalues per cycle against the oracle's loop (1e-9)."""
+    from oracle.scf import scf
+    s = _setup(gpu_ctx, "g", 12, 48)
+    sysm = s["sysm"]
+    occ0 = np.full(sysm.nst, 2.0)
+    nitscf, nite = 3, 2
+    cd = to_dev(s["psi"])
+    hist, occ, eig, mu = s["ham"].scf_run_smeared(
+        s["vb"], cd.data_ptr(), s["basis"].ngw, sysm.nst, occ0, 0.0, 0.0, sysm.omega,
+        s["vd"].data_ptr(), s["pd"].data_ptr(), nitscf, nite, sysm.ecut)
+    gpu_ctx.sync()
+    _, h_ref, cyc = scf(s["maps"], s["vmaps"], [s["sp"]], s["psi"], occ0, sysm.omega,
+                        s["vion"], s["rhops"], nitscf, nite, sysm.ecut, diag=True)
+    for i in range(nitscf):
+        for j in range(nite):
+            assert abs(hist[i, j, 0] - h_ref[i][j]["etotal"]) <= 1e-9, (i, j)
+        assert np.max(np.abs(eig[i] - cyc[i]["eig"])) <= 1e-9, i
+    assert np.array_equal(occ, occ0) and np.all(mu == 0.0)
+    _close(s)
