@@ -59,6 +59,9 @@ def parse():
     p.add_argument("--scf-nite", type=int, default=3)
     p.add_argument("--scf-diag", action="store_true",
                    help="subspace diagonalisation after every SCF cycle (wf_diag T)")
+    p.add_argument("--scf-alpha", type=float, default=0.5, help="charge mixing coefficient")
+    p.add_argument("--scf-ndim", type=int, default=3, help="Anderson history (<= 8)")
+    p.add_argument("--scf-kerker", type=float, default=10.0, help="Kerker rc (bohr)")
     p.add_argument("--batch-streams", type=int, default=4,
                    help="contexts (HIP streams) the 12 batch cells are spread over")
     p.add_argument("--no-tune", action="store_true",
@@ -420,10 +423,13 @@ def scf_bench(args, ctx, basis, ham, sysm, species, psi0):
         if args.scf_diag:
             hist = ham.scf_run_smeared(vb, psi.ptr, ngw, nst, occ, 2.0 * nst, 0.0, sysm.omega,
                                        vion.ptr, rhops.ptr, args.scf_cycles, args.scf_nite,
-                                       sysm.ecut, flags_inner=fi, flags_last=QBH_ALL)[0]
+                                       sysm.ecut, alpha=args.scf_alpha, ndim=args.scf_ndim,
+                                       rc_kerker=args.scf_kerker, flags_inner=fi,
+                                       flags_last=QBH_ALL)[0]
         else:
             hist = ham.scf_run(vb, psi.ptr, ngw, nst, occ, sysm.omega, vion.ptr, rhops.ptr,
-                               args.scf_cycles, args.scf_nite, sysm.ecut, flags_inner=fi,
+                               args.scf_cycles, args.scf_nite, sysm.ecut, alpha=args.scf_alpha,
+                               ndim=args.scf_ndim, rc_kerker=args.scf_kerker, flags_inner=fi,
                                flags_last=QBH_ALL)
         synchronize()
         t1 = time.perf_counter()
@@ -442,7 +448,8 @@ def scf_bench(args, ctx, basis, ham, sysm, species, psi0):
     a, b = runs["fp64"], runs["precision"]
     out = {"workload": f"Si{sysm.na} Γ, Si HSCV LDA (rcps {species.rcps:.2f}), {ngw} PWs, "
                        f"{nst} bands, grid {sysm.np[0]}^3, {args.scf_cycles} SCF cycles x "
-                       f"{args.scf_nite} PSDA iterations, Anderson ndim 3, Kerker rc 10"
+                       f"{args.scf_nite} PSDA iterations, Anderson ndim {args.scf_ndim}, "
+                       f"mixing {args.scf_alpha}, Kerker rc {args.scf_kerker:g}"
                        + (", subspace diagonalisation each cycle" if args.scf_diag else ""),
            "esr_Ha": esr, "eself_Ha": eself,
            "dE_per_atom_Ha": abs(b["etotal_Ha"] - a["etotal_Ha"]) / sysm.na,

@@ -107,6 +107,40 @@ struct UnitCell {
   }
 };
 
+// Norm-conserving KB species from a species XML file (Species::initialize, Species.cc:74-673);
+// rcps <= 0 takes the width AtomSet::set_rcps picks for the cell and cutoff (AtomSet.cc:1102)
+class Species {
+ public:
+  Species(const std::string& path, double rcps) {
+    if (qbh_species_create_xml(path.c_str(), rcps, &h_) != QBH_OK)
+      throw std::runtime_error("qbh_species_create_xml: " + path);
+    info();
+  }
+  Species(const std::string& path, const UnitCell& cell, double ecut) {
+    double z = 0.0;
+    qbh_species probe = nullptr;
+    if (qbh_species_create_xml(path.c_str(), 1.0, &probe) != QBH_OK)
+      throw std::runtime_error("qbh_species_create_xml: " + path);
+    qbh_species_info(probe, &z, nullptr, nullptr, nullptr, nullptr, nullptr);
+    qbh_species_destroy(probe);
+    double rc = 0.0;
+    if (qbh_species_auto_rcps(z, cell.a, ecut, &rc) != QBH_OK ||
+        qbh_species_create_xml(path.c_str(), rc, &h_) != QBH_OK)
+      throw std::runtime_error("species set-up failed: " + path);
+    info();
+  }
+  ~Species() { if (h_) qbh_species_destroy(h_); }
+  Species(const Species&) = delete;
+  Species& operator=(const Species&) = delete;
+  qbh_species get() const { return h_; }
+  double zval = 0.0, rcps = 0.0, eself = 0.0;
+  int nproj = 0;
+
+ private:
+  void info() { qbh_species_info(h_, &zval, &rcps, &eself, &nproj, nullptr, nullptr); }
+  qbh_species h_ = nullptr;
+};
+
 // Plane-wave basis: rods in the caller's order (Basis::rod_h/k/lmin/size); when built
 // from a cell and cutoff it restates Basis::resize through qbh_rods_build.
 class Basis {
@@ -298,6 +332,17 @@ class EnergyFunctional {
                                  wt.data(), twnl.data(), tau.data(),
                                  ft_.basis().cell().volume),
           ft_.context().get(), "qbh_ham_set_projectors");
+    na_ = na;
+  }
+  // the same tables made on the device from species (update_twnl), re-tabled at |k+G-A|
+  // by set_vector_potential; tau host, 3 per atom, species after species
+  void set_species(const std::vector<const Species*>& sps, const std::vector<int>& na,
+                   const std::vector<double>& tau) {
+    std::vector<qbh_species> h;
+    for (const Species* s : sps) h.push_back(s->get());
+    check(qbh_ham_set_species(h_, (int)h.size(), h.data(), na.data(), tau.data(),
+                              ft_.basis().cell().volume),
+          ft_.context().get(), "qbh_ham_set_species");
     na_ = na;
   }
   // non-local ionic forces, NonLocalPotential::energy(sd, ..., compute_forces=true, fion)
