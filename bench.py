@@ -53,15 +53,17 @@ def parse():
                    help="skip the Si64 TDDFT forward/adjoint section (differentiation form)")
     p.add_argument("--no-scf", action="store_true",
                    help="skip the Si512 SCF section (precision form against FP64)")
-    p.add_argument("--scf-cycles", type=int, default=20)
+    p.add_argument("--scf-cycles", type=int, default=120)
     p.add_argument("--smeared-cycles", type=int, default=8,
                    help="Fermi-smeared SCF cycles of the Al500 section")
     p.add_argument("--scf-nite", type=int, default=3)
-    p.add_argument("--scf-diag", action="store_true",
-                   help="subspace diagonalisation after every SCF cycle (wf_diag T)")
+    p.add_argument("--scf-no-diag", dest="scf_diag", action="store_false",
+                   help="no subspace diagonalisation after the SCF cycles (default: wf_diag T)")
     p.add_argument("--scf-alpha", type=float, default=0.5, help="charge mixing coefficient")
     p.add_argument("--scf-ndim", type=int, default=3, help="Anderson history (<= 8)")
     p.add_argument("--scf-kerker", type=float, default=10.0, help="Kerker rc (bohr)")
+    p.add_argument("--scf-ecutprec", type=float, default=1.5,
+                   help="PSDA preconditioner cutoff in Hartree (0: the wavefunction cutoff)")
     p.add_argument("--batch-streams", type=int, default=4,
                    help="contexts (HIP streams) the 12 batch cells are spread over")
     p.add_argument("--no-tune", action="store_true",
@@ -392,12 +394,14 @@ def main():
 
 
 def scf_bench(args, ctx, basis, ham, sysm, species, psi0):
-    """The precision form at its own size: the BOSampleStepper SCF loop (PSDA + Anderson /
-    Kerker charge mixing, qbh_scf_run) on Si512 with the Si HSCV LDA pseudopotential (v_ion,
-    rho_ps from its local part, Ewald esr - eself added to etotal), from the benchmark's
-    orthonormal start, run twice: all Hψ in FP64, and FP32-bulk Hψ with the last inner
-    iteration of every cycle in FP64.  Reports etotal of both, ΔE per atom, the density L1
-    difference and the non-local force difference of the final states, and times."""
+    """The precision form at its own size: the BOSampleStepper SCF loop (PSDA with the
+    preconditioner cutoff --scf-ecutprec, Anderson / Kerker charge mixing, the subspace
+    diagonalisation after every cycle) on Si512 with the Si HSCV LDA pseudopotential
+    (v_ion, rho_ps from its local part, Ewald esr - eself added to etotal), from the
+    benchmark's orthonormal start, run twice: all Hψ in FP64, and FP32-bulk Hψ with the
+    last inner iteration of every cycle in FP64.  Reports etotal of both, ΔE per atom, the
+    density L1 difference and the non-local force difference of the final states, the last
+    cycle's energy change (convergence) and times."""
     from qball_amd import QBH_ALL, QBH_FP32, Basis, capi, ewald_esr, local_tables
     from qball_amd.devmem import DeviceArray, synchronize
     ngw, nst = sysm.ngw, sysm.nst
@@ -411,6 +415,7 @@ def scf_bench(args, ctx, basis, ham, sysm, species, psi0):
     esr = ewald_esr(sysm.a, [species], [sysm.na], sysm.tau)
     eself = sysm.na * species.eself
     occ = np.full(nst, 2.0)
+    ecp = args.scf_ecutprec if args.scf_ecutprec > 0.0 else sysm.ecut
     start = psi0.to_numpy()
     psi = DeviceArray(psi0.nbytes)
     N = int(np.prod(sysm.np))
@@ -423,12 +428,12 @@ def scf_bench(args, ctx, basis, ham, sysm, species, psi0):
         if args.scf_diag:
             hist = ham.scf_run_smeared(vb, psi.ptr, ngw, nst, occ, 2.0 * nst, 0.0, sysm.omega,
                                        vion.ptr, rhops.ptr, args.scf_cycles, args.scf_nite,
-                                       sysm.ecut, alpha=args.scf_alpha, ndim=args.scf_ndim,
+                                       ecp, alpha=args.scf_alpha, ndim=args.scf_ndim,
                                        rc_kerker=args.scf_kerker, flags_inner=fi,
                                        flags_last=QBH_ALL)[0]
         else:
             hist = ham.scf_run(vb, psi.ptr, ngw, nst, occ, sysm.omega, vion.ptr, rhops.ptr,
-                               args.scf_cycles, args.scf_nite, sysm.ecut, alpha=args.scf_alpha,
+                               args.scf_cycles, args.scf_nite, ecp, alpha=args.scf_alpha,
                                ndim=args.scf_ndim, rc_kerker=args.scf_kerker, flags_inner=fi,
                                flags_last=QBH_ALL)
         synchronize()
@@ -449,7 +454,8 @@ def scf_bench(args, ctx, basis, ham, sysm, species, psi0):
     out = {"workload": f"Si{sysm.na} Γ, Si HSCV LDA (rcps {species.rcps:.2f}), {ngw} PWs, "
                        f"{nst} bands, grid {sysm.np[0]}^3, {args.scf_cycles} SCF cycles x "
                        f"{args.scf_nite} PSDA iterations, Anderson ndim {args.scf_ndim}, "
-                       f"mixing {args.scf_alpha}, Kerker rc {args.scf_kerker:g}"
+                       f"mixing {args.scf_alpha}, Kerker rc {args.scf_kerker:g}, "
+                       f"preconditioner cutoff {ecp:g} Ha"
                        + (", subspace diagonalisation each cycle" if args.scf_diag else ""),
            "esr_Ha": esr, "eself_Ha": eself,
            "dE_per_atom_Ha": abs(b["etotal_Ha"] - a["etotal_Ha"]) / sysm.na,
