@@ -12,15 +12,18 @@
 //
 // Workgroup: 256 threads = 4 waves in a 2 x 2 arrangement, a 128 x 128 tile of C (each
 // wave 64 x 64 = 4 x 4 blocks of the 16 x 16 x 4 MFMA, 64 accumulators per lane), the
-// reduction in stages (16 deep in FP64, 32 in FP32) double-buffered through LDS: each
-// stage's global loads are in flight while the previous stage's MFMAs run (64 or 128 per
-// wave), one barrier per stage, two workgroups per CU.  Loads
-// past the ends of the operands read a zero pair instead (no branches, no fix-up), so the
-// compiler keeps all of a stage's loads in flight together.  The fnl product splits the
-// reduction into slices (the 2048 x 1024 output of 512 Si atoms has only 128 tiles for
-// 256 CUs), each writing its own partial, which the finish kernel sums.  Blocks are
-// numbered so that the workgroups that run together on one XCD (blocks are dealt
-// round-robin to the 8 XCDs) share a tile of the projector table in that XCD's L2.
+// reduction in stages 16 deep double-buffered through LDS: each stage's global loads are
+// in flight while the previous stage's MFMAs run (64 per wave), one barrier per stage, two
+// workgroups per CU in FP64 and three in FP32.  The operands are read through buffer
+// descriptors (a scalar base per stage and a fixed 32-bit offset per lane, which frees the
+// 64-bit address registers of pointer loads: FP64 9.7 / 9.9 ms against 10.1 / 10.0 ms,
+// FP32 4.8 / 4.8 against 5.6 / 5.3 at the 512-atom shapes below).  Loads past the ends of
+// the operands read a zero pair instead (no branches, no fix-up), so the compiler keeps all
+// of a stage's loads in flight together.  The fnl product splits the reduction into slices
+// (the 2048 x 1024 output of 512 Si atoms has only 128 tiles for 256 CUs), each writing its
+// own partial, which the finish kernel sums.  Blocks are numbered so that the workgroups
+// that run together on one XCD (blocks are dealt round-robin to the 8 XCDs) share a tile of
+// the projector table in that XCD's L2.
 //
 // Lane layout of the 16 x 16 x 4 MFMA (measured on gfx950): lane i supplies A[i mod 16][i / 16]
 // and B[i / 16][i mod 16]; lane i receives in accumulator j the result row 4 (i / 16) + j
@@ -42,9 +45,10 @@ namespace {
 constexpr int kBR = 128, kBC = 128, kThreads = 256;
 
 // stage depth (reduction elements per LDS stage) and waves per SIMD, per element type
-// (measured at the 512-atom Si shapes, F = anl^T psi / hpsi += anl F: FP64 16-deep stages
-// at two waves per SIMD 10.0 / 10.0 ms, 32-deep at one wave 12.4 / 12.6 ms; FP32 32-deep at
-// two waves 5.6 / 5.3 ms, 16-deep at three the same, at four (spilling) 8.2 / 8.7 ms)
+// (measured at the 512-atom Si shapes, F = anl^T psi / hpsi += anl F, buffer loads: FP64
+// 16-deep stages at two waves per SIMD 9.7 / 9.9 ms; FP32 16-deep 4.8 / 4.8 ms, 32-deep
+// 4.8 / 5.1 ms.  Pointer loads: FP64 32-deep at one wave 12.4 / 12.6 ms, FP32 at four waves
+// per SIMD (spilling) 8.2 / 8.7 ms)
 #ifndef QBH_GEMM_BQ64
 #define QBH_GEMM_BQ64 16
 #endif
@@ -52,10 +56,13 @@ constexpr int kBR = 128, kBC = 128, kThreads = 256;
 #define QBH_GEMM_WPE64 2
 #endif
 #ifndef QBH_GEMM_BQ32
-#define QBH_GEMM_BQ32 32
+#define QBH_GEMM_BQ32 16
 #endif
 #ifndef QBH_GEMM_WPE32
 #define QBH_GEMM_WPE32 2
+#endif
+#ifndef QBH_GEMM_BUF
+#define QBH_GEMM_BUF 1
 #endif
 constexpr int kBQ = QBH_GEMM_BQ64 > QBH_GEMM_BQ32 ? QBH_GEMM_BQ64 : QBH_GEMM_BQ32;  // slice unit
 
@@ -133,6 +140,42 @@ __device__ inline typename Mf<T>::v2 ld2(const T* __restrict__ x, long i, long l
   return v;
 }
 
+// Buffer addressing: a descriptor per operand and stage (base, record count in bytes),
+// one 32-bit byte offset per lane and element pair.  A lane whose pair lies outside the
+// operand takes an offset past the record count, and the load returns zeros; the record
+// count is the stage's byte extent, below kOutside.
+constexpr unsigned kOutside = 0x80000000u;
+constexpr int kRsrcWord3 = 0x00020000;  // raw dword access (gfx9 family)
+
+template <typename T>
+__device__ inline typename Mf<T>::v2 bload2(__amdgpu_buffer_rsrc_t r, unsigned off) {
+  using v2 = typename Mf<T>::v2;
+  if constexpr (sizeof(T) == 8) {
+    typedef unsigned u4 __attribute__((ext_vector_type(4)));
+    const u4 u = __builtin_amdgcn_raw_buffer_load_b128(r, off, 0, 0);
+    typedef double d2 __attribute__((ext_vector_type(2)));
+    const d2 d = __builtin_bit_cast(d2, u);
+    v2 o;
+    o.x = d.x;
+    o.y = d.y;
+    return o;
+  } else {
+    typedef unsigned u2 __attribute__((ext_vector_type(2)));
+    const u2 u = __builtin_amdgcn_raw_buffer_load_b64(r, off, 0, 0);
+    typedef float f2 __attribute__((ext_vector_type(2)));
+    const f2 f = __builtin_bit_cast(f2, u);
+    v2 o;
+    o.x = f.x;
+    o.y = f.y;
+    return o;
+  }
+}
+
+__device__ inline __amdgpu_buffer_rsrc_t rsrc(const void* base, unsigned bytes) {
+  return __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(base), (short)0, (int)bytes,
+                                           kRsrcWord3);
+}
+
 // YC: Y is c-contiguous.  OUTC: the output is c-contiguous (O[c + r ldo]), else
 // r-contiguous (O[r + c ldo]).  XV/YV: the operand's element pairs are aligned vectors that
 // never straddle the end of the range.
@@ -140,7 +183,11 @@ __device__ inline typename Mf<T>::v2 ld2(const T* __restrict__ x, long i, long l
 // Reduction order: within a stage of BQ, lane group g supplies reduction index
 // (BQ / 4) g + j in MFMA step j, for the X and the Y fragment alike, so each lane reads its
 // X values of a stage as BQ / 4 consecutive elements of one row.
-template <typename T, bool YC, bool OUTC, bool XV, bool YV>
+//
+// BUF (with XV and YV): the operands are read through buffer descriptors based at the
+// stage's first element, with per-lane offsets fixed for the whole reduction; the host
+// selects it when one stage of either operand spans less than kOutside bytes.
+template <typename T, bool YC, bool OUTC, bool XV, bool YV, bool BUF>
 __global__ void __launch_bounds__(kThreads, Mf<T>::WPE) k_gemm(Args a) {
   using M = Mf<T>;
   using v2 = typename M::v2;
@@ -183,8 +230,55 @@ __global__ void __launch_bounds__(kThreads, Mf<T>::WPE) k_gemm(Args a) {
   const int wr = wave >> 1, wc = wave & 1;
   const int l16 = lane & 15, lg = lane >> 4;
 
+  // buffer addressing: byte offsets of this lane's pairs from the stage base, and where
+  // each pair sits along the reduction (qx, qy) and across the tile (rows, columns)
+  unsigned xo[NP], yo[NP];
+  int xq[NP], yq[NP];
+  bool xin[NP], yin[NP];
+  unsigned xext = 0, yext = 0;
+  if constexpr (BUF) {
+#pragma unroll
+    for (int it = 0; it < NP; ++it) {
+      const int v = it * kThreads + tid;
+      const int row = v / PPR, qv = (v % PPR) * 2;
+      xo[it] = (unsigned)((qv + (long)row * a.ldx) * (long)sizeof(T));
+      xq[it] = qv;
+      xin[it] = r0 + row < a.R;
+      if (YC) {
+        const int qr = v / PPC, cv = (v % PPC) * 2;
+        yo[it] = (unsigned)((cv + (long)qr * a.ldy) * (long)sizeof(T));
+        yq[it] = qr;
+        yin[it] = c0 + cv < a.C;  // C even: the pair is whole
+      } else {
+        const int col = v / PPR, qv2 = (v % PPR) * 2;
+        yo[it] = (unsigned)((qv2 + (long)col * a.ldy) * (long)sizeof(T));
+        yq[it] = qv2;
+        yin[it] = c0 + col < a.C;
+      }
+    }
+    const long rows = min(kBR, a.R - r0), cols = min(kBC, a.C - c0);
+    xext = (unsigned)(((rows - 1) * a.ldx + BQ) * (long)sizeof(T));
+    yext = YC ? (unsigned)(((BQ - 1) * a.ldy + cols) * (long)sizeof(T))
+              : (unsigned)(((cols - 1) * a.ldy + BQ) * (long)sizeof(T));
+  }
+
   // staging: NP element pairs of X and of Y per thread and stage
   auto gload = [&](v2 (&xs)[NP], v2 (&ys)[NP], long qb) {
+    if constexpr (BUF) {
+      const long left = q1 - qb;  // reduction elements left from the stage start
+      const __amdgpu_buffer_rsrc_t rx = rsrc(X + qb + (long)r0 * a.ldx, xext);
+      const __amdgpu_buffer_rsrc_t ry =
+          rsrc(YC ? Y + c0 + qb * a.ldy : Y + qb + (long)c0 * a.ldy, yext);
+#pragma unroll
+      for (int it = 0; it < NP; ++it)
+        xs[it] = bload2<T>(rx, xin[it] && xq[it] + 2 <= left ? xo[it] : kOutside);
+#pragma unroll
+      for (int it = 0; it < NP; ++it) {
+        const bool in = YC ? yq[it] < left : yq[it] + 2 <= left;
+        ys[it] = bload2<T>(ry, yin[it] && in ? yo[it] : kOutside);
+      }
+      return;
+    }
 #pragma unroll
     for (int it = 0; it < NP; ++it) {
       const int v = it * kThreads + tid;
@@ -307,14 +401,21 @@ void launch(const Args& a, hipStream_t st) {
   // c-contiguous ones) are even (slice ends are multiples of 16)
   const bool xv = (a.ldx % 2) == 0 && (a.Q % 2) == 0;
   const bool yv = (a.ldy % 2) == 0 && ((YC ? (long)a.C : a.Q) % 2) == 0;
-  if (xv && yv)
-    k_gemm<T, YC, OUTC, true, true><<<grid, kThreads, 0, st>>>(a);
+  // buffer addressing: a stage of each operand spans less than kOutside bytes
+  constexpr long BQ = Mf<T>::BQ;
+  const long xspan = ((kBR - 1) * a.ldx + BQ) * (long)sizeof(T);
+  const long yspan = (YC ? (BQ - 1) * a.ldy + kBC : (kBC - 1) * a.ldy + BQ) * (long)sizeof(T);
+  const bool buf = QBH_GEMM_BUF && xv && yv && xspan < (long)kOutside && yspan < (long)kOutside;
+  if (buf)
+    k_gemm<T, YC, OUTC, true, true, true><<<grid, kThreads, 0, st>>>(a);
+  else if (xv && yv)
+    k_gemm<T, YC, OUTC, true, true, false><<<grid, kThreads, 0, st>>>(a);
   else if (xv)
-    k_gemm<T, YC, OUTC, true, false><<<grid, kThreads, 0, st>>>(a);
+    k_gemm<T, YC, OUTC, true, false, false><<<grid, kThreads, 0, st>>>(a);
   else if (yv)
-    k_gemm<T, YC, OUTC, false, true><<<grid, kThreads, 0, st>>>(a);
+    k_gemm<T, YC, OUTC, false, true, false><<<grid, kThreads, 0, st>>>(a);
   else
-    k_gemm<T, YC, OUTC, false, false><<<grid, kThreads, 0, st>>>(a);
+    k_gemm<T, YC, OUTC, false, false, false><<<grid, kThreads, 0, st>>>(a);
 }
 
 }  // namespace
