@@ -64,6 +64,9 @@ constexpr int kBR = 128, kBC = 128, kThreads = 256;
 #ifndef QBH_GEMM_BUF
 #define QBH_GEMM_BUF 1
 #endif
+#ifndef QBH_GEMM_DIRECT
+#define QBH_GEMM_DIRECT 1
+#endif
 constexpr int kBQ = QBH_GEMM_BQ64 > QBH_GEMM_BQ32 ? QBH_GEMM_BQ64 : QBH_GEMM_BQ32;  // slice unit
 
 
@@ -393,6 +396,158 @@ __global__ void __launch_bounds__(kThreads, Mf<T>::WPE) k_gemm(Args a) {
   }
 }
 
+// Direct form (FP64 enl product): no LDS.  Each wave loads its own fragments from global
+// memory through buffer descriptors into a ring of register stages (8 reduction elements
+// each), two stages in flight while a third feeds the MFMAs, so there is no barrier and no
+// LDS store.  The Y operand (the projector table, c-contiguous) is read one reduction row at
+// a time, 16 lanes taking 16 consecutive elements of one 128-byte line.  At the 512-atom
+// shape: 9.2 ms against 9.9 ms for the LDS form; F, whose q-contiguous rows it would read in
+// 64-byte pieces, and FP32 are slower in this form (12.2 / 8.7 / 5.8 against 9.9 / 4.8 /
+// 4.8 ms) and keep the LDS form.
+template <typename T>
+__device__ inline T bload1(__amdgpu_buffer_rsrc_t r, unsigned off) {
+  if constexpr (sizeof(T) == 8) {
+    return __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(r, off, 0, 0));
+  } else {
+    return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(r, off, 0, 0));
+  }
+}
+
+template <typename T, bool YC, bool OUTC>
+__global__ void __launch_bounds__(kThreads, 2) k_gemm_direct(Args a) {
+  using M = Mf<T>;
+  using v2 = typename M::v2;
+  using acc_t = typename M::acc;
+  constexpr int QG = sizeof(T) == 8 ? 2 : 4;  // reduction values per lane, block and stage
+  constexpr int BQ = 4 * QG;
+  constexpr int NP = QG / 2;
+  constexpr unsigned SZ = sizeof(T);
+  struct Stage {
+    T x[4][QG];
+    T y[4][QG];
+  };
+  const int nb8 = gridDim.x >> 3;
+  const int bid = blockIdx.x;
+  const int wid = (bid & 7) * nb8 + (bid >> 3);
+  if (wid >= a.nblocks) return;
+  const int tiles = a.tiles_r * a.tiles_c;
+  const int s = wid / tiles;
+  const int t = wid - s * tiles;
+  int tr, tc;
+  if (a.r_fastest) {
+    tr = t % a.tiles_r;
+    tc = t / a.tiles_r;
+  } else {
+    tc = t % a.tiles_c;
+    tr = t / a.tiles_c;
+  }
+  const int r0 = tr * kBR, c0 = tc * kBC;
+  const long q0 = (long)s * a.qslice;
+  const long q1 = min(a.Q, q0 + a.qslice);
+  const T* __restrict__ X = static_cast<const T*>(a.X);
+  const T* __restrict__ Y = static_cast<const T*>(a.Y);
+  const int tid = threadIdx.x;
+  const int lane = tid & 63, wave = __builtin_amdgcn_readfirstlane(tid >> 6);
+  const int wr = wave >> 1, wc = wave & 1;
+  const int l16 = lane & 15, lg = lane >> 4;
+  const int rw0 = r0 + wr * 64, cw0 = c0 + wc * 64;
+  const int rows = a.R - rw0, cols = a.C - cw0;
+  unsigned xo[4], yo[4];
+  bool xin[4], yin[4];
+#pragma unroll
+  for (int b = 0; b < 4; ++b) {
+    xo[b] = (unsigned)(((long)(b * 16 + l16) * a.ldx + QG * lg) * SZ);
+    xin[b] = b * 16 + l16 < rows;
+    yo[b] = YC ? (unsigned)(((long)(b * 16 + l16) + (long)QG * lg * a.ldy) * SZ)
+               : (unsigned)(((long)(b * 16 + l16) * a.ldy + QG * lg) * SZ);
+    yin[b] = b * 16 + l16 < cols;
+  }
+  const unsigned xext = rows > 0 ? (unsigned)(((long)(min(64, rows) - 1) * a.ldx + BQ) * SZ) : 0u;
+  const unsigned yext =
+      cols > 0 ? (YC ? (unsigned)(((long)(BQ - 1) * a.ldy + min(64, cols)) * SZ)
+                     : (unsigned)(((long)(min(64, cols) - 1) * a.ldy + BQ) * SZ))
+               : 0u;
+
+  auto load = [&](Stage& st, long qb) {
+    const long left = q1 - qb;
+    const __amdgpu_buffer_rsrc_t rx = rsrc(X + qb + (long)rw0 * a.ldx, xext);
+    const __amdgpu_buffer_rsrc_t ry =
+        rsrc(YC ? Y + cw0 + qb * a.ldy : Y + qb + (long)cw0 * a.ldy, yext);
+#pragma unroll
+    for (int b = 0; b < 4; ++b)
+#pragma unroll
+      for (int p = 0; p < NP; ++p) {
+        const bool ok = xin[b] && QG * lg + 2 * p + 2 <= left;
+        const v2 v = bload2<T>(rx, ok ? xo[b] + 2 * p * SZ : kOutside);
+        st.x[b][2 * p] = v.x;
+        st.x[b][2 * p + 1] = v.y;
+      }
+#pragma unroll
+    for (int b = 0; b < 4; ++b)
+#pragma unroll
+      for (int p = 0; p < NP; ++p) {
+        if (YC) {
+          const int k0 = QG * lg + 2 * p;
+          const bool ok0 = yin[b] && k0 < left, ok1 = yin[b] && k0 + 1 < left;
+          st.y[b][2 * p] = bload1<T>(ry, ok0 ? yo[b] + (unsigned)(2 * p * a.ldy * SZ) : kOutside);
+          st.y[b][2 * p + 1] =
+              bload1<T>(ry, ok1 ? yo[b] + (unsigned)((2 * p + 1) * a.ldy * SZ) : kOutside);
+        } else {
+          const bool ok = yin[b] && QG * lg + 2 * p + 2 <= left;
+          const v2 v = bload2<T>(ry, ok ? yo[b] + 2 * p * SZ : kOutside);
+          st.y[b][2 * p] = v.x;
+          st.y[b][2 * p + 1] = v.y;
+        }
+      }
+  };
+  acc_t acc[4][4];
+#pragma unroll
+  for (int i = 0; i < 4; ++i)
+#pragma unroll
+    for (int j = 0; j < 4; ++j) acc[i][j] = acc_t{0, 0, 0, 0};
+  auto mma = [&](const Stage& st) {
+#pragma unroll
+    for (int j = 0; j < QG; ++j)
+#pragma unroll
+      for (int bi = 0; bi < 4; ++bi)
+#pragma unroll
+        for (int bj = 0; bj < 4; ++bj) {
+          acc[bi][bj] = M::op(st.x[bi][j], st.y[bj][j], acc[bi][bj]);
+        }
+  };
+  // loads are unconditional: past q1 every lane is outside and reads zeros
+  Stage s0, s1, s2;
+  load(s0, q0);
+  load(s1, q0 + BQ);
+  for (long qb = q0; qb < q1; qb += 3 * BQ) {
+    load(s2, qb + 2 * BQ);
+    mma(s0);
+    load(s0, qb + 3 * BQ);
+    mma(s1);
+    load(s1, qb + 4 * BQ);
+    mma(s2);
+  }
+  T* O = static_cast<T*>(a.O) + (long)s * a.ostride;
+#pragma unroll
+  for (int bi = 0; bi < 4; ++bi) {
+    const int rb = rw0 + bi * 16;
+#pragma unroll
+    for (int bj = 0; bj < 4; ++bj) {
+      const int c = cw0 + bj * 16 + l16;
+      if (c >= a.C) continue;
+      const acc_t v = acc[bi][bj];
+#pragma unroll
+      for (int j = 0; j < 4; ++j) {
+        const int r = rb + M::row(lg, j);
+        if (r < a.R) {
+          T* o = OUTC ? O + c + (long)r * a.ldo : O + r + (long)c * a.ldo;
+          *o = a.accumulate ? *o + v[j] : v[j];
+        }
+      }
+    }
+  }
+}
+
 template <typename T, bool YC, bool OUTC>
 void launch(const Args& a, hipStream_t st) {
   const unsigned grid = (unsigned)((a.nblocks + 7) / 8 * 8);
@@ -406,6 +561,12 @@ void launch(const Args& a, hipStream_t st) {
   const long xspan = ((kBR - 1) * a.ldx + BQ) * (long)sizeof(T);
   const long yspan = (YC ? (BQ - 1) * a.ldy + kBC : (kBC - 1) * a.ldy + BQ) * (long)sizeof(T);
   const bool buf = QBH_GEMM_BUF && xv && yv && xspan < (long)kOutside && yspan < (long)kOutside;
+  if constexpr (YC && sizeof(T) == 8) {
+    if (buf && QBH_GEMM_DIRECT) {
+      k_gemm_direct<T, YC, OUTC><<<grid, kThreads, 0, st>>>(a);
+      return;
+    }
+  }
   if (buf)
     k_gemm<T, YC, OUTC, true, true, true><<<grid, kThreads, 0, st>>>(a);
   else if (xv && yv)
