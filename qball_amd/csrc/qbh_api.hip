@@ -267,12 +267,12 @@ extern "C" int qbh_ham_create(qbh_ctx ctx, qbh_basis basis, qbh_ham* out) {
 
 static void free_proj(Ham* h) {
   for (void* p : {(void*)h->col_l, (void*)h->col_tw, (void*)h->col_atom, (void*)h->col_fac,
-                  (void*)h->twnl, (void*)h->tau, h->anl64, h->anl32})
+                  (void*)h->twnl, (void*)h->tau, h->anl64, h->anl32, h->anlc64, h->anlc32})
     if (p) hipFree(p);
   h->col_l = h->col_tw = h->col_atom = nullptr;
   h->col_fac = h->twnl = h->tau = nullptr;
-  h->anl64 = h->anl32 = nullptr;
-  h->anl64_ok = h->anl32_ok = false;
+  h->anl64 = h->anl32 = h->anlc64 = h->anlc32 = nullptr;
+  h->anl64_ok = h->anl32_ok = h->anlc64_ok = h->anlc32_ok = false;
   h->P = 0;
   h->col_atom_host.clear();
 }

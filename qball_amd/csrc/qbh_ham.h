@@ -32,6 +32,11 @@ struct Ham {
   void* anl64 = nullptr;          // Γ: 2ngw x P real; complex: ngw x P complex
   void* anl32 = nullptr;
   bool anl64_ok = false, anl32_ok = false;
+  // complex bases, this library's matrix-core products: 2P rows of 2ngw reals, row 2p the
+  // anl row of projector p, row 2p + 1 the row times i (built from anl when it changes)
+  void* anlc64 = nullptr;
+  void* anlc32 = nullptr;
+  bool anlc64_ok = false, anlc32_ok = false;
   // scratch (grown on demand)
   void* f64 = nullptr;            // F (P x nst), real (Γ) or complex
   size_t f64_bytes = 0;
@@ -57,11 +62,11 @@ struct Ham {
   // tuning (qbh_ham_set_tuning / qbh_ham_autotune): 0 / -1 = built-in choice
   int nsplit = 0;    // K slices of the projector contraction
   int overlap = -1;  // projector GEMMs on the side stream (1), after the transforms (0)
-  // projector GEMM implementation: kOwnGemm = this library's matrix-core kernels (real Γ
-  // view; F over K slices); 0 = rocBLAS (F over nsplit K slices); 1..16 = hipBLASLt with
-  // the n-th algorithm its heuristic lists for the shape (F over the full K; n past the end
-  // of the list: the last one); shapes hipBLASLt lists nothing for (complex) and complex
-  // bases asked for kOwnGemm use rocBLAS; -1 = built-in choice (nl_builtin_algo)
+  // projector GEMM implementation: kOwnGemm = this library's matrix-core kernels (real
+  // view of Γ, or the combined table of a complex basis; F over K slices); 0 = rocBLAS (F
+  // over nsplit K slices); 1..16 = hipBLASLt with the n-th algorithm its heuristic lists
+  // for the shape (F over the full K; n past the end of the list: the last one); shapes
+  // hipBLASLt lists nothing for (complex) use rocBLAS; -1 = built-in choice (nl_builtin_algo)
   int fnl_algo = -1, enl_algo = -1;
   void* etrs = nullptr;           // ETRS step: psi(t+dt/2), psi~(t+dt), rho, rhog
   size_t etrs_bytes = 0;
@@ -96,8 +101,7 @@ int nl_forces(Ham* h, const double* psi, long ld, int nst, const double* occ, in
 int nl_lt_candidates(Ham* h, int which, int nst, long ld, long ldh, int fp32);
 // fnl_algo / enl_algo value of the matrix-core kernels of qbh_gemm.hip
 constexpr int kOwnGemm = 17;
-// the value the built-in choice (-1) of fnl_algo / enl_algo stands for: kOwnGemm for the
-// real Γ view, rocBLAS (0) for complex bases
+// the value the built-in choice (-1) of fnl_algo / enl_algo stands for: kOwnGemm
 int nl_builtin_algo(Ham* h);
 int blas_handle(Ctx* c, void** out);
 // frees a context's rocBLAS handle and hipBLASLt state (qbh_nl.hip)

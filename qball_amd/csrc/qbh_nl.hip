@@ -120,7 +120,7 @@ int nl_lt_builtin_algo(Ctx* c) {
 
 // Built-in choice of both projector products: the matrix-core kernels of qbh_gemm.hip for
 // the real Γ view, rocBLAS for complex bases.
-int nl_builtin_algo(Ham* h) { return h->b->g.real ? kOwnGemm : 0; }
+int nl_builtin_algo(Ham*) { return kOwnGemm; }
 
 Ctx* ctx_of(qbh_ctx c);
 
@@ -397,6 +397,7 @@ static int ensure_anl(Ham* h, bool fp32) {
                                                h->tau, h->b->kpgx, h->A[0], h->A[1], h->A[2],
                                                static_cast<double*>(h->anl64));
     h->anl64_ok = true;
+    h->anlc64_ok = false;
   }
   if (fp32 && !h->anl32_ok) {
     if (!h->anl32 && hipMalloc(&h->anl32, n * sizeof(float)) != hipSuccess) {
@@ -410,12 +411,66 @@ static int ensure_anl(Ham* h, bool fp32) {
                                               h->tau, h->b->kpgx, h->A[0], h->A[1], h->A[2],
                                               static_cast<float*>(h->anl32));
     h->anl32_ok = true;
+    h->anlc32_ok = false;
   }
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) {
     c->last_error = std::string("anl: ") + hipGetErrorString(e);
     return QBH_ERR_DEVICE;
   }
+  return 0;
+}
+
+// Complex bases on this library's (real) matrix-core kernels: with rows 2p = beta_p and
+// 2p + 1 = i beta_p of 2ngw interleaved reals, F = beta^H psi is one real product (row 2p:
+// Re = br pr + bi pi, row 2p + 1: Im = br pi - bi pr, which lands in the interleaved
+// complex layout of F), and psi += beta F is one real product over q = (p, re/im) with the
+// same rows (Re = br Fr - bi Fi, Im = bi Fr + br Fi).
+template <typename T>
+__global__ void __launch_bounds__(256)
+k_anl_pair(int ngw, const T* __restrict__ anl, T* __restrict__ out) {
+  const int ig = blockIdx.x * 256 + threadIdx.x;
+  if (ig >= ngw) return;
+  const long p = blockIdx.y;
+  const T re = anl[p * 2 * ngw + 2 * ig], im = anl[p * 2 * ngw + 2 * ig + 1];
+  T* a = out + 2 * p * 2 * ngw + 2 * ig;
+  a[0] = re;
+  a[1] = im;
+  a[2L * ngw] = -im;
+  a[2L * ngw + 1] = re;
+}
+
+// the combined table of a complex basis (after ensure_anl)
+static int ensure_anlc(Ham* h, bool fp32) {
+  Ctx* c = h->ctx;
+  const int ngw = h->b->g.ngw;
+  const size_t n = 4 * (size_t)ngw * h->P;
+  void*& tab = fp32 ? h->anlc32 : h->anlc64;
+  bool& ok = fp32 ? h->anlc32_ok : h->anlc64_ok;
+  if (ok) return 0;
+  if (t_capturing) {
+    c->last_error = "projector table would be built during graph capture";
+    return QBH_ERR_ALLOC;
+  }
+  if (!tab && hipMalloc(&tab, n * (fp32 ? sizeof(float) : sizeof(double))) != hipSuccess) {
+    tab = nullptr;
+    c->last_error = "combined projector table allocation failed";
+    return QBH_ERR_ALLOC;
+  }
+  TimedLaunch tl(c, "anl");
+  dim3 grid((ngw + 255) / 256, h->P);
+  if (fp32)
+    k_anl_pair<float><<<grid, 256, 0, c->stream>>>(ngw, static_cast<const float*>(h->anl32),
+                                                   static_cast<float*>(tab));
+  else
+    k_anl_pair<double><<<grid, 256, 0, c->stream>>>(ngw, static_cast<const double*>(h->anl64),
+                                                    static_cast<double*>(tab));
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) {
+    c->last_error = std::string("anl pair: ") + hipGetErrorString(e);
+    return QBH_ERR_DEVICE;
+  }
+  ok = true;
   return 0;
 }
 
@@ -561,8 +616,14 @@ static int fnl_stage(Ham* h, rocblas_handle bh, const double* psi, long ld, int 
   const int e = cplx ? 2 : 1;
   // hipBLASLt algorithm chosen for this shape (nullptr: rocBLAS)
   o->sel = h->fnl_algo >= 0 ? h->fnl_algo : nl_builtin_algo(h);
-  if (o->sel == kOwnGemm && !real) o->sel = 0;  // complex bases: rocBLAS
   const bool own = o->sel == kOwnGemm;
+  // own kernels: rows Ro of the real view (complex: rows (p, re/im) of the combined table)
+  // over Qo = 2 ngw reals
+  const int Ro = real ? P : 2 * P;
+  const long Qo = 2L * ngw;
+  if (own && cplx) {
+    if (int r = ensure_anlc(h, fp32)) return r;
+  }
   LtPlan* fplan = (o->sel > 0 && !own) ? nl_plan(h, 0, nst, ld, ldh, fp32) : nullptr;
   const int falgo = fplan ? std::min<int>(o->sel, (int)fplan->cand.size()) - 1 : -1;
   // K slices of kc elements; the slice count actually written (ceil(K / kc)) can be below
@@ -570,10 +631,10 @@ static int fnl_stage(Ham* h, rocblas_handle bh, const double* psi, long ld, int 
   // hipBLASLt contracts the full K in one product.
   // the own kernel cuts K into its own slices (multiples of its 16-row stage)
   const int want = fplan ? 1
-                         : own ? mfma_slices(P, nst, Kelem, h->nsplit)
+                         : own ? mfma_slices(Ro, nst, Qo, h->nsplit)
                                : (h->nsplit > 0 ? h->nsplit : fnl_splits(P, nst, Kelem));
-  const long kc = own ? mfma_slice_len(Kelem, want) : (Kelem + want - 1) / want;
-  const int nsplit = (int)((Kelem + kc - 1) / kc);
+  const long kc = own ? mfma_slice_len(Qo, want) : (Kelem + want - 1) / want;
+  const int nsplit = own ? (int)((Qo + kc - 1) / kc) : (int)((Kelem + kc - 1) / kc);
   const long pstride = (long)P * nst * e;
   const size_t fsz = (size_t)pstride * nsplit;
   if (!own && !bh) {
@@ -589,7 +650,8 @@ static int fnl_stage(Ham* h, rocblas_handle bh, const double* psi, long ld, int 
     {
       TimedLaunch tl(c, "fnl_gemm");
       if (own) {
-        if (mfma_gemm_qq(c->stream, false, P, nst, Kelem, anl, Kelem, psi, 2 * ld, F, P, nsplit,
+        const double* tab = real ? anl : static_cast<const double*>(h->anlc64);
+        if (mfma_gemm_qq(c->stream, false, Ro, nst, Qo, tab, Qo, psi, 2 * ld, F, Ro, nsplit,
                          pstride))
           return gemm_fail(c, "fnl");
       } else if (int r = fplan ? lt_gemm<double>(c, fplan, falgo, anl, psi, F, 0.0)
@@ -623,8 +685,8 @@ static int fnl_stage(Ham* h, rocblas_handle bh, const double* psi, long ld, int 
     {
       TimedLaunch tl(c, "fnl_gemm");
       if (own) {
-        if (mfma_gemm_qq(c->stream, true, P, nst, Kelem, anl, Kelem, p32, Kelem, F, P, nsplit,
-                         pstride))
+        const float* tab = real ? anl : static_cast<const float*>(h->anlc32);
+        if (mfma_gemm_qq(c->stream, true, Ro, nst, Qo, tab, Qo, p32, Qo, F, Ro, nsplit, pstride))
           return gemm_fail(c, "fnl");
       } else if (int r = fplan ? lt_gemm<float>(c, fplan, falgo, anl, p32, F, 0.0f)
                                : fnl_gemm<float>(c, bh, cplx, P, nst, Kelem, anl, Kelem, p32,
@@ -664,8 +726,12 @@ int nl_apply(Ham* h, const double* psi, long ld, int nst, const double* occ_host
 
   const long Kelem = real ? 2L * ngw : (long)ngw;
   int esel = h->enl_algo >= 0 ? h->enl_algo : nl_builtin_algo(h);
-  if (esel == kOwnGemm && !real) esel = 0;  // complex bases: rocBLAS
   const bool own = esel == kOwnGemm;
+  const int Ro = real ? P : 2 * P;  // own kernels: reduction rows (see fnl_stage)
+  const long Qo = 2L * ngw;
+  if (own && cplx) {
+    if (int r = ensure_anlc(h, fp32 != 0)) return r;
+  }
   LtPlan* eplan = (esel > 0 && !own) ? nl_plan(h, 1, nst, ld, ldh, fp32 != 0) : nullptr;
   const int ealgo = eplan ? std::min<int>(esel, (int)eplan->cand.size()) - 1 : -1;
   if (!own && !eplan) {
@@ -678,7 +744,8 @@ int nl_apply(Ham* h, const double* psi, long ld, int nst, const double* occ_host
     const double beta = overwrite ? 0.0 : 1.0;
     if (own) {
       // hpsi[k, n] (+)= sum_p anl[k, p] F[p, n], taken transposed (r = band, c = row)
-      if (mfma_gemm_qc(c->stream, false, nst, (int)Kelem, P, F, P, anl, Kelem, hpsi, 2 * ldh,
+      const double* tab = real ? anl : static_cast<const double*>(h->anlc64);
+      if (mfma_gemm_qc(c->stream, false, nst, (int)Qo, Ro, F, Ro, tab, Qo, hpsi, 2 * ldh,
                        !overwrite))
         return gemm_fail(c, "enl");
     } else if (eplan) {
@@ -702,8 +769,8 @@ int nl_apply(Ham* h, const double* psi, long ld, int nst, const double* occ_host
     {
       TimedLaunch tl(c, "enl_gemm");
       if (own) {
-        if (mfma_gemm_qc(c->stream, true, nst, (int)Kelem, P, F, P, anl, Kelem, t32, Kelem,
-                         false))
+        const float* tab = real ? anl : static_cast<const float*>(h->anlc32);
+        if (mfma_gemm_qc(c->stream, true, nst, (int)Qo, Ro, F, Ro, tab, Qo, t32, Qo, false))
           return gemm_fail(c, "enl");
       } else if (eplan) {
         if (int r = lt_gemm<float>(c, eplan, ealgo, anl, F, t32, 0.0f)) return r;
@@ -823,11 +890,13 @@ static int dfnl_forces(Ham* h, rocblas_handle bh, const double* psi, long ld, in
   const bool own = sel == kOwnGemm;
   LtPlan* dplan = (sel > 0 && !own) ? nl_plan(h, 0, nst, ngw, ngw, fp32) : nullptr;
   const int dalgo = dplan ? std::min<int>(sel, (int)dplan->cand.size()) - 1 : -1;
+  const int Ro = real ? P : 2 * P;  // own kernels: rows of the real view (see fnl_stage)
+  const long Qo = 2L * ngw;
   const int want = dplan ? 1
-                         : own ? mfma_slices(P, nst, Kelem, h->nsplit)
+                         : own ? mfma_slices(Ro, nst, Qo, h->nsplit)
                                : (h->nsplit > 0 ? h->nsplit : fnl_splits(P, nst, Kelem));
-  const long kc = own ? mfma_slice_len(Kelem, want) : (Kelem + want - 1) / want;
-  const int nsplit = (int)((Kelem + kc - 1) / kc);
+  const long kc = own ? mfma_slice_len(Qo, want) : (Kelem + want - 1) / want;
+  const int nsplit = own ? (int)((Qo + kc - 1) / kc) : (int)((Kelem + kc - 1) / kc);
   if (!own && !dplan && !bh) {
     if (int r = blas_rocblas(c, &bh)) return r;
   }
@@ -849,7 +918,9 @@ static int dfnl_forces(Ham* h, rocblas_handle bh, const double* psi, long ld, in
     {
       TimedLaunch tl(c, "dfnl_gemm");
       if (own) {
-        if (mfma_gemm_qq(c->stream, fp32, P, nst, Kelem, anl, Kelem, dpsi, Kelem, dF, P, nsplit,
+        const T* tab = real ? anl
+                            : static_cast<const T*>(fp32 ? h->anlc32 : h->anlc64);
+        if (mfma_gemm_qq(c->stream, fp32, Ro, nst, Qo, tab, Qo, dpsi, Qo, dF, Ro, nsplit,
                          pstride))
           return gemm_fail(c, "dfnl");
       } else if (int r = dplan ? lt_gemm<T>(c, dplan, dalgo, anl, dpsi, dF, T(0))
@@ -947,8 +1018,7 @@ extern "C" int qbh_ctx_library_info(qbh_ctx ctx, char* buf, int cap) {
   hipRuntimeGetVersion(&hv);
   const int lv = lt_ready(c) ? static_cast<LtState*>(c->lt)->version : 0;
   std::snprintf(buf, (size_t)cap,
-                "hip %d rocblas %s hipblaslt %d projector-gemm-builtin mfma (complex bases: "
-                "rocblas)", hv, rb, lv);
+                "hip %d rocblas %s hipblaslt %d projector-gemm-builtin mfma", hv, rb, lv);
   return QBH_OK;
 }
 

@@ -184,7 +184,7 @@ extern "C" int qbh_ham_autotune(qbh_ham ham, const double* psi_dev, long ld, int
     for (int which = 0; nl && which < 2; ++which) {
       const int na = nl_lt_candidates(h, which, nst, ld, ldh, (fl & QBH_FP32) ? 1 : 0);
       std::vector<int> vals;
-      if (g.real) vals.push_back(kOwnGemm);  // the matrix-core kernels of this library
+      vals.push_back(kOwnGemm);  // the matrix-core kernels of this library
       for (int v = 0; v <= std::min(na, kMaxAlgo); ++v) vals.push_back(v);
       if (vals.size() > 1) {
         // start from the value the built-in choice stands for, so it is not timed twice
