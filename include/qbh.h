@@ -354,7 +354,8 @@ int qbh_scf_run_smeared(qbh_ham ham, qbh_basis vbasis, double* psi_dev, long ld,
  *             this library's matrix-core kernels (17; F over K slices), by rocBLAS (0) or
  *             by the n-th algorithm hipBLASLt lists for the shape (1..16; F over the full
  *             K; n past the end of the list: the last; complex bases, which hipBLASLt
- *             does not cover, use rocBLAS for 1..16); -1 = built-in: 17 */
+ *             does not cover, use rocBLAS for 1..16); -1 = built-in: 17, except rocBLAS
+ *             for the enl product of a complex basis */
 int qbh_ham_set_tuning(qbh_ham ham, const char* knob, int value);
 int qbh_ham_get_tuning(qbh_ham ham, const char* knob, int* value);
 /* Times candidate knob values for this Hψ shape (reps applications each, HIP events on the

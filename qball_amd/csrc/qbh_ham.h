@@ -101,8 +101,9 @@ int nl_forces(Ham* h, const double* psi, long ld, int nst, const double* occ, in
 int nl_lt_candidates(Ham* h, int which, int nst, long ld, long ldh, int fp32);
 // fnl_algo / enl_algo value of the matrix-core kernels of qbh_gemm.hip
 constexpr int kOwnGemm = 17;
-// the value the built-in choice (-1) of fnl_algo / enl_algo stands for: kOwnGemm
-int nl_builtin_algo(Ham* h);
+// the value the built-in choice (-1) of fnl_algo (which 0) / enl_algo (which 1) stands
+// for: kOwnGemm, except rocBLAS (0) for the enl product of a complex basis
+int nl_builtin_algo(Ham* h, int which);
 int blas_handle(Ctx* c, void** out);
 // frees a context's rocBLAS handle and hipBLASLt state (qbh_nl.hip)
 void blas_destroy(Ctx* c);

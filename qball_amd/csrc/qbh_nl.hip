@@ -120,7 +120,11 @@ int nl_lt_builtin_algo(Ctx* c) {
 
 // Built-in choice of both projector products: the matrix-core kernels of qbh_gemm.hip for
 // the real Γ view, rocBLAS for complex bases.
-int nl_builtin_algo(Ham*) { return kOwnGemm; }
+int nl_builtin_algo(Ham* h, int which) {
+  // complex bases: rocBLAS for Hψ += βF, which ran 0.145 against 0.216 ms per call (Si64
+  // TD, beside the concurrent transforms) while F = βᴴψ ran 0.137 against 0.226 ms here
+  return (which == 1 && !h->b->g.real) ? 0 : kOwnGemm;
+}
 
 Ctx* ctx_of(qbh_ctx c);
 
@@ -615,7 +619,7 @@ static int fnl_stage(Ham* h, rocblas_handle bh, const double* psi, long ld, int 
   const long Kelem = real ? 2L * ngw : (long)ngw;  // contraction length in elements
   const int e = cplx ? 2 : 1;
   // hipBLASLt algorithm chosen for this shape (nullptr: rocBLAS)
-  o->sel = h->fnl_algo >= 0 ? h->fnl_algo : nl_builtin_algo(h);
+  o->sel = h->fnl_algo >= 0 ? h->fnl_algo : nl_builtin_algo(h, 0);
   const bool own = o->sel == kOwnGemm;
   // own kernels: rows Ro of the real view (complex: rows (p, re/im) of the combined table)
   // over Qo = 2 ngw reals
@@ -725,7 +729,7 @@ int nl_apply(Ham* h, const double* psi, long ld, int nst, const double* occ_host
   if (int r = fnl_stage(h, nullptr, psi, ld, nst, ldh, fp32 != 0, occ, &fs)) return r;
 
   const long Kelem = real ? 2L * ngw : (long)ngw;
-  int esel = h->enl_algo >= 0 ? h->enl_algo : nl_builtin_algo(h);
+  int esel = h->enl_algo >= 0 ? h->enl_algo : nl_builtin_algo(h, 1);
   const bool own = esel == kOwnGemm;
   const int Ro = real ? P : 2 * P;  // own kernels: reduction rows (see fnl_stage)
   const long Qo = 2L * ngw;
@@ -1018,7 +1022,8 @@ extern "C" int qbh_ctx_library_info(qbh_ctx ctx, char* buf, int cap) {
   hipRuntimeGetVersion(&hv);
   const int lv = lt_ready(c) ? static_cast<LtState*>(c->lt)->version : 0;
   std::snprintf(buf, (size_t)cap,
-                "hip %d rocblas %s hipblaslt %d projector-gemm-builtin mfma", hv, rb, lv);
+                "hip %d rocblas %s hipblaslt %d projector-gemm-builtin mfma (complex-basis enl: "
+                "rocblas)", hv, rb, lv);
   return QBH_OK;
 }
 

@@ -189,7 +189,7 @@ extern "C" int qbh_ham_autotune(qbh_ham ham, const double* psi_dev, long ld, int
       if (vals.size() > 1) {
         // start from the value the built-in choice stands for, so it is not timed twice
         int& knob = which ? h->enl_algo : h->fnl_algo;
-        if (knob < 0) knob = nl_builtin_algo(h);
+        if (knob < 0) knob = nl_builtin_algo(h, which);
         plan.push_back({6 + which, vals});
       }
     }
