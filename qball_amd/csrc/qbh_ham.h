@@ -113,6 +113,9 @@ void* grow(void*& p, size_t& have, size_t need);
 int species_table(qbh_species sp, Ctx* c, Basis* b, const double* A, int grad, double* out,
                   long dstride);
 int species_nproj(qbh_species sp);
+// combined table of complex projector rows (rows x ngw complex, ld ngw) for the matrix-core
+// products: 2 rows x 2ngw reals, row 2p = row p, row 2p + 1 = i row p (qbh_nl.hip)
+int anl_pair_c64(Ctx* c, int ngw, int rows, const double* anl, double* out);
 // re-tables the Hamiltonian's twnl rows from its species at its vector potential
 int retable_twnl(Ham* h);
 // complex projector table (ngw x P, ld ngw) of the Hamiltonian's columns built from the rows

@@ -444,6 +444,17 @@ k_anl_pair(int ngw, const T* __restrict__ anl, T* __restrict__ out) {
   a[2L * ngw + 1] = re;
 }
 
+int anl_pair_c64(Ctx* c, int ngw, int rows, const double* anl, double* out) {
+  dim3 grid((ngw + 255) / 256, rows);
+  k_anl_pair<double><<<grid, 256, 0, c->stream>>>(ngw, anl, out);
+  const hipError_t e = hipGetLastError();
+  if (e != hipSuccess) {
+    c->last_error = std::string("anl pair: ") + hipGetErrorString(e);
+    return QBH_ERR_DEVICE;
+  }
+  return 0;
+}
+
 // the combined table of a complex basis (after ensure_anl)
 static int ensure_anlc(Ham* h, bool fp32) {
   Ctx* c = h->ctx;

@@ -25,7 +25,6 @@
 //   dF_d = danl_d^H x, danl_d built like anl from d twnl / dq_d;
 //   through v = V_Hxc[rho]: nu = f_xc(rho) mu + K_H mu, lambda += (f_b N/Omega) V_loc[nu] psi.
 #include <hip/hip_runtime.h>
-#include <rocblas/rocblas.h>
 
 #include <algorithm>
 #include <cmath>
@@ -35,6 +34,7 @@
 #include <vector>
 
 #include "../../include/qbh.h"
+#include "qbh_gemm.h"
 #include "qbh_ham.h"
 
 namespace qbh {
@@ -256,6 +256,16 @@ __global__ void k_band_axpy(int ngw, int nst, const double* __restrict__ f, doub
 
 inline unsigned nb256(long n) { return (unsigned)((n + 255) / 256); }
 
+// out[i] = sum over the S slices of part (slice stride n), i < n
+__global__ void __launch_bounds__(256)
+k_sum_slices(long n, int S, const double* __restrict__ part, double* __restrict__ out) {
+  for (long i = blockIdx.x * 256L + threadIdx.x; i < n; i += (long)gridDim.x * 256) {
+    double a = part[i];
+    for (int s = 1; s < S; ++s) a += part[(long)s * n + i];
+    out[i] = a;
+  }
+}
+
 }  // namespace
 
 struct qbh_td_s {
@@ -293,6 +303,10 @@ struct qbh_td_s {
   bool nl_adj = false;
   int nrows = 0;
   double *tw = nullptr, *dtw = nullptr, *anl4 = nullptr, *fz4 = nullptr, *fu4 = nullptr;
+  // combined table of anl4 for the matrix-core products (8P rows of 2ngw reals) and their
+  // K-slice partials (nsl slices of 8P x order*nst reals)
+  double *anl8 = nullptr, *fsl = nullptr;
+  int nsl = 1;
   bool tab_ok = false;  // anl4 holds the tables at tab_A
   double tab_A[3] = {0.0, 0.0, 0.0};
   size_t blk() const { return 2 * (size_t)ngw * nst; }
@@ -381,7 +395,6 @@ static int cross(qbh_td_s* t, const double* zk, const double* u, const double* A
   Ham* h = t->h;
   const int P = h->P;
   const bool nl = t->nl_adj && (whole_h_flags(t->flags) & QBH_NONLOCAL);
-  rocblas_handle bh = nullptr;
   if (nl && !(t->tab_ok && A[0] == t->tab_A[0] && A[1] == t->tab_A[1] && A[2] == t->tab_A[2])) {
     // tables at this exponential's A: twnl rows, their gradient, anl and danl_d (kept for
     // the next exponential under the same A)
@@ -397,22 +410,20 @@ static int cross(qbh_td_s* t, const double* zk, const double* u, const double* A
     if (int r = build_anl_c64(h, t->tw, A, t->anl4)) return r;
     for (int d = 0; d < 3; ++d)
       if (int r = build_anl_c64(h, t->dtw + d * rs, A, t->anl4 + (1 + d) * cs)) return r;
+    if (int r = anl_pair_c64(cx, t->ngw, 4 * P, t->anl4, t->anl8)) return r;
     t->tab_ok = true;
     std::copy(A, A + 3, t->tab_A);
   }
-  if (nl) {
-    void* hv = nullptr;
-    if (int r = blas_handle(cx, &hv)) return tfail(cx, r, "rocBLAS handle creation failed");
-    bh = static_cast<rocblas_handle>(hv);
-  }
-  // F = [anl | danl_x | danl_y | danl_z]^H x for the order blocks of x at once
-  auto products = [&](const double* x, double* out) {
-    const rocblas_double_complex one(1.0, 0.0), zero(0.0, 0.0);
-    return rocblas_zgemm(bh, rocblas_operation_conjugate_transpose, rocblas_operation_none,
-                         4 * P, o * t->nst, t->ngw, &one,
-                         reinterpret_cast<const rocblas_double_complex*>(t->anl4), t->ngw,
-                         reinterpret_cast<const rocblas_double_complex*>(x), t->ngw, &zero,
-                         reinterpret_cast<rocblas_double_complex*>(out), 4 * P);
+  // F = [anl | danl_x | danl_y | danl_z]^H x for the order blocks of x at once: one real
+  // matrix-core product over the combined table (row 2r: Re, row 2r + 1: Im of row r of
+  // F, which is the interleaved complex layout), K slices summed into out
+  auto products = [&](const double* x, double* out) -> bool {
+    const long R8 = 8L * P, C = (long)o * t->nst, Q = 2L * t->ngw;
+    if (mfma_gemm_qq(cx->stream, false, (int)R8, (int)C, Q, t->anl8, Q, x, Q, t->fsl, R8,
+                     t->nsl, R8 * C))
+      return false;
+    k_sum_slices<<<nb256(R8 * C), 256, 0, cx->stream>>>(R8 * C, t->nsl, t->fsl, out);
+    return true;
   };
   const int nb = t->nb_cross;
   double* fz = t->grids;
@@ -429,9 +440,8 @@ static int cross(qbh_td_s* t, const double* zk, const double* u, const double* A
     k_combo<<<nb256(nc), 256, 0, cx->stream>>>(nc, cb, t->ubuf + k * B);
   }
   if (nl) {
-    if (products(zk, t->fz4) != rocblas_status_success ||
-        products(t->ubuf, t->fu4) != rocblas_status_success)
-      return tfail(cx, QBH_ERR_BLAS, "td projector products failed");
+    if (!products(zk, t->fz4) || !products(t->ubuf, t->fu4))
+      return tfail(cx, QBH_ERR_ARG, "td projector products: invalid shape");
     k_nl_dA_part<<<NBLK, 256, 0, cx->stream>>>(P, o * t->nst, h->col_fac, t->fz4, t->fu4,
                                                t->part);
     k_final<<<1, 256, 0, cx->stream>>>(NBLK, 3, t->part, -2.0, 1, gA3);
@@ -505,9 +515,12 @@ static int ensure_memory(qbh_td_s* t) {
   t->nrows = 0;
   for (qbh_species sp : h->sps) t->nrows += species_nproj(sp);
   const size_t rows_d = (size_t)t->nrows * t->ngw;
+  t->nsl = h->P > 0 ? mfma_slices(8 * h->P, t->order * t->nst, 2L * t->ngw, 16) : 1;
   const size_t tabs[] = {rows_d, 3 * rows_d, 8 * (size_t)t->ngw * h->P,
                          8 * (size_t)h->P * t->nst * t->order,
-                         8 * (size_t)h->P * t->nst * t->order};
+                         8 * (size_t)h->P * t->nst * t->order,
+                         16 * (size_t)t->ngw * h->P,
+                         (size_t)t->nsl * 8 * h->P * t->nst * t->order};
   t->tab_ok = false;
   if (t->nl_adj)
     for (size_t s : tabs) add(s);
@@ -557,6 +570,8 @@ static int ensure_memory(qbh_td_s* t) {
     t->anl4 = carve(p, tabs[2]);
     t->fz4 = carve(p, tabs[3]);
     t->fu4 = carve(p, tabs[4]);
+    t->anl8 = carve(p, tabs[5]);
+    t->fsl = carve(p, tabs[6]);
   }
   hipMemcpyAsync(t->fj, t->jfac.data(), t->nst * sizeof(double), hipMemcpyHostToDevice,
                  t->c->stream);
