@@ -3,7 +3,8 @@ hpsi += anl F taken transposed) against the oracle's non-local term
 (NonLocalPotential::energy, NonLocalPotential.cc:2041-2171) on shapes that exercise their
 edges: projector counts that are odd (element pairs of F not aligned) or not a multiple of
 the 128-wide tile, band counts past one tile and not a multiple of it, K slices from one to
-many, FP64 and FP32, overwriting and accumulating outputs.
+many, FP64 and FP32, overwriting and accumulating outputs; on Γ (the real view) and on a
+complex basis (the combined table of rows beta_p and i beta_p).
 
 FP64 bar: relative 2-norm error <= 1e-12 (Enl <= 1e-10 relative); FP32 form <= 2e-6.
 """
@@ -111,4 +112,49 @@ def test_projector_products_accumulate(gpu_ctx, si64, fp32):
     got, _, psi, occ = _run(gpu_ctx, si64, [sp], 0x1 | 0x8 | (0x10 if fp32 else 0),
                             accumulate_from=base)
     ref, _ = _ref(si64, [sp], psi, occ)
+    assert rel_err(got - base, ref) <= (2e-6 if fp32 else 1e-12)
+
+
+# ---- complex bases: the same kernels over the combined table (rows beta_p and i beta_p) ----
+
+@pytest.fixture(scope="module")
+def si8c():
+    from qball_amd.workload import make_system
+    return make_system(ncell=1, ecut_ry=25.0, nbands=300, seed=33, complex_basis=True)
+
+
+@pytest.mark.parametrize("natoms,lproj", [(5, (0, 1, 2)), (47, (0, 1, 1)), (3, (0,))])
+@pytest.mark.parametrize("fp32", [False, True])
+def test_complex_projector_products_ragged(gpu_ctx, si8c, natoms, lproj, fp32):
+    """Complex basis, both products on the matrix-core kernels: 2P = 30, 282 and 6 table
+    rows (not multiples of the 128-row tile), 300 bands (the last band tile partial)."""
+    assert not si8c.real
+    sp = _species(si8c, natoms, lproj, seed=100 + natoms)
+    flags = 0x1 | (0x10 if fp32 else 0)
+    got, enl, psi, occ = _run(gpu_ctx, si8c, [sp], flags)
+    ref, enl_ref = _ref(si8c, [sp], psi, occ)
+    assert rel_err(got, ref) <= (2e-6 if fp32 else 1e-12), (natoms, lproj, fp32)
+    assert abs(enl - enl_ref) <= (1e-5 if fp32 else 1e-10) * abs(enl_ref)
+
+
+@pytest.mark.parametrize("nst", [1, 129])
+@pytest.mark.parametrize("nsplit", [0, 1, 7])
+def test_complex_projector_products_band_counts_and_slices(gpu_ctx, si8c, nst, nsplit):
+    """Complex basis: one band and one band past a tile, K in one slice or several."""
+    _, _, sp = oracle_for(si8c)
+    got, enl, psi, occ = _run(gpu_ctx, si8c, [sp], 0x1, nst=nst, nsplit=nsplit)
+    ref, enl_ref = _ref(si8c, [sp], psi, occ)
+    assert rel_err(got, ref) <= 1e-12, (nst, nsplit)
+    assert abs(enl - enl_ref) <= 1e-10 * abs(enl_ref)
+
+
+@pytest.mark.parametrize("fp32", [False, True])
+def test_complex_projector_products_accumulate(gpu_ctx, si8c, fp32):
+    """Complex basis: QBH_ACCUMULATE adds the non-local term to what the output holds."""
+    _, _, sp = oracle_for(si8c)
+    rng = np.random.default_rng(7)
+    base = rng.standard_normal(si8c.psi.shape) + 1j * rng.standard_normal(si8c.psi.shape)
+    got, _, psi, occ = _run(gpu_ctx, si8c, [sp], 0x1 | 0x8 | (0x10 if fp32 else 0),
+                            accumulate_from=base)
+    ref, _ = _ref(si8c, [sp], psi, occ)
     assert rel_err(got - base, ref) <= (2e-6 if fp32 else 1e-12)
