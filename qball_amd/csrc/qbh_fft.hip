@@ -181,19 +181,22 @@ k_zpass_bwd(GridDev g, const double* __restrict__ psi, long ld, int nst, int ban
     cplx<T>* line = buf + le * LD;
     for (int i0 = sub; i0 < size; i0 += 16 * KZ) {
       double a[KZ], b[KZ], c[KZ], d[KZ];
+      // branch-free: every lane loads an element of its rod (past the end: the last one,
+      // never stored), so all KZ loads are in flight together
 #pragma unroll
       for (int k = 0; k < KZ; ++k) {
-        const int i = i0 + 16 * k;
-        a[k] = b[k] = c[k] = d[k] = 0.0;
-        if (i < size) {
-          a[k] = p1[2 * (first + i)];
-          b[k] = p1[2 * (first + i) + 1];
-          if (has2) {
-            c[k] = p2[2 * (first + i)];
-            d[k] = p2[2 * (first + i) + 1];
-          }
+        const long e = 2 * (long)(first + min(i0 + 16 * k, size - 1));
+        a[k] = p1[e];
+        b[k] = p1[e + 1];
+        c[k] = d[k] = 0.0;
+        if (MODE == SLOT_PAIR) {  // p2 is p1 for a lone last band; its values are dropped
+          c[k] = p2[e];
+          d[k] = p2[e + 1];
         }
       }
+      if (!has2)
+#pragma unroll
+        for (int k = 0; k < KZ; ++k) c[k] = d[k] = 0.0;
 #pragma unroll
       for (int k = 0; k < KZ; ++k) {
         const int i = i0 + 16 * k;
@@ -311,14 +314,36 @@ k_ypass_fwd(GridDev g, const cplx<T>* __restrict__ t2, cplx<T>* __restrict__ t1)
   }
   __syncthreads();
   cplx<T>* wbuf = buf + wave * LPW * LD;
-  for (int ll = 0; ll < LPW; ++ll) {
-    const int li = wave * LPW + ll;
-    const int dz = li % ZB, dl = li / ZB;
-    const int z = z0 + dz, lx = lx0 + dl;
-    const bool ok = z < np2 && lx < lx1;
-    const cplx<T>* src = t2 + (((long)slot * np2 + (ok ? z : 0)) * nlx + (ok ? lx : 0)) * np1;
-    for (int y = lane; y < np1; y += 64)
-      wbuf[ll * LD + PD::idx(y)] = ok ? src[y] : cplx<T>{T(0), T(0)};
+  {
+    // the wave's LPW lines (KY elements per lane each) are all loaded before any is stored
+    // to LDS, so no load waits on the one before it (lines past the grid read a valid
+    // line and store zeros)
+    // elements per lane: a register transform's length is N1 N2 (which can exceed NMAX),
+    // a generic one's at most NMAX
+    constexpr int KY = ((N1 > 0 ? N1 * N2 : NMAX) + 63) / 64;
+    cplx<T> v[LPW][KY];
+    bool in[LPW][KY];
+#pragma unroll
+    for (int ll = 0; ll < LPW; ++ll) {
+      const int li = wave * LPW + ll;
+      const int dz = li % ZB, dl = li / ZB;
+      const int z = z0 + dz, lx = lx0 + dl;
+      const bool ok = z < np2 && lx < lx1;
+      const cplx<T>* src = t2 + (((long)slot * np2 + (ok ? z : 0)) * nlx + (ok ? lx : 0)) * np1;
+#pragma unroll
+      for (int k = 0; k < KY; ++k) {
+        const int y = lane + 64 * k;
+        in[ll][k] = ok && y < np1;
+        v[ll][k] = src[in[ll][k] ? y : 0];
+      }
+    }
+#pragma unroll
+    for (int ll = 0; ll < LPW; ++ll)
+#pragma unroll
+      for (int k = 0; k < KY; ++k) {
+        const int y = lane + 64 * k;
+        if (y < np1) wbuf[ll * LD + PD::idx(y)] = in[ll][k] ? v[ll][k] : cplx<T>{T(0), T(0)};
+      }
   }
   wave_sync();
   line_fft<-1, EPL, N1, N2, LPW>(wbuf, LPW, LD, g.ax[1], tw, lane);
