@@ -257,14 +257,44 @@ k_ypass_bwd(GridDev g, const cplx<T>* __restrict__ t1, cplx<T>* __restrict__ t2)
   __syncthreads();
   const int c0 = g.line_start[lx0], c1 = g.line_start[lx1];
   const int nwork = (c1 - c0) * ZB;
-  for (int f = threadIdx.x; f < nwork; f += NTHR) {
-    const int dz = f % ZB;
-    const int c = c0 + f / ZB;
-    const int z = z0 + dz;
-    if (z < np2) {
-      int dl = 0;
-      while (lx0 + dl + 1 < lx1 && c >= g.line_start[lx0 + dl + 1]) ++dl;
-      buf[(dz + ZB * dl) * LD + PD::idx(g.col_y[c])] = t1[((long)slot * g.ncol + c) * np2 + z];
+  if constexpr (sizeof(T) == 4) {
+    // FP32: KG elements per thread and round, all loaded (with their column indices) before
+    // any is stored; past the end a thread reloads the last element and stores nothing
+    // (at Si512 1.64 -> 1.49 ms per Hψ; in FP64 the same form measured 3.33 -> 3.40-3.55 ms)
+    constexpr int KG = 4;
+    for (int f0 = threadIdx.x; f0 < nwork; f0 += NTHR * KG) {
+      cplx<T> v[KG];
+      int col[KG], cc[KG];
+#pragma unroll
+      for (int k = 0; k < KG; ++k) {
+        const int f = min(f0 + k * NTHR, nwork - 1);
+        const int c = c0 + f / ZB;
+        const int z = min(z0 + f % ZB, np2 - 1);
+        cc[k] = c;
+        col[k] = g.col_y[c];
+        v[k] = t1[((long)slot * g.ncol + c) * np2 + z];
+      }
+#pragma unroll
+      for (int k = 0; k < KG; ++k) {
+        const int f = f0 + k * NTHR;
+        const int dz = f % ZB;
+        if (f < nwork && z0 + dz < np2) {
+          int dl = 0;
+          while (lx0 + dl + 1 < lx1 && cc[k] >= g.line_start[lx0 + dl + 1]) ++dl;
+          buf[(dz + ZB * dl) * LD + PD::idx(col[k])] = v[k];
+        }
+      }
+    }
+  } else {
+    for (int f = threadIdx.x; f < nwork; f += NTHR) {
+      const int dz = f % ZB;
+      const int c = c0 + f / ZB;
+      const int z = z0 + dz;
+      if (z < np2) {
+        int dl = 0;
+        while (lx0 + dl + 1 < lx1 && c >= g.line_start[lx0 + dl + 1]) ++dl;
+        buf[(dz + ZB * dl) * LD + PD::idx(g.col_y[c])] = t1[((long)slot * g.ncol + c) * np2 + z];
+      }
     }
   }
   __syncthreads();
