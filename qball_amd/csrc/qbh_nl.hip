@@ -233,8 +233,18 @@ k_fnl_finish_real(int P, int nst, T* __restrict__ F, int nsplit, long pstride,
     const int n = (int)(i / P);
     const T a0 = anl[(long)p * astride];
     const T c0 = psi[(long)n * bstride];
+    // split-K partial sums, in slice order; four slices' loads in flight at a time
     T f = F[i];
-    for (int k = 1; k < nsplit; ++k) f += F[k * pstride + i];  // split-K partial sums
+    int k = 1;
+    for (; k + 3 < nsplit; k += 4) {
+      const T a0 = F[k * pstride + i], a1 = F[(k + 1) * pstride + i];
+      const T a2 = F[(k + 2) * pstride + i], a3 = F[(k + 3) * pstride + i];
+      f += a0;
+      f += a1;
+      f += a2;
+      f += a3;
+    }
+    for (; k < nsplit; ++k) f += F[k * pstride + i];
     f = f + a0 * (T(-0.5) * c0);
     f = T(2) * f;
     const double fp = fac[p];
@@ -264,8 +274,23 @@ k_fnl_finish_cplx(int P, int nst, T* __restrict__ F, int nsplit, long pstride,
     const int p = (int)(i % P);
     const int n = (int)(i / P);
     const double fp = fac[p];
+    // split-K partial sums, in slice order; four slices' loads in flight at a time
     T xr = F[2 * i], xi = F[2 * i + 1];
-    for (int k = 1; k < nsplit; ++k) {  // split-K partial sums
+    int k = 1;
+    for (; k + 3 < nsplit; k += 4) {
+      T r[4], m[4];
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        r[u] = F[(k + u) * pstride + 2 * i];
+        m[u] = F[(k + u) * pstride + 2 * i + 1];
+      }
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        xr += r[u];
+        xi += m[u];
+      }
+    }
+    for (; k < nsplit; ++k) {
       xr += F[k * pstride + 2 * i];
       xi += F[k * pstride + 2 * i + 1];
     }
