@@ -43,8 +43,9 @@ def parse():
     p.add_argument("--warmup", type=int, default=2)
     p.add_argument("--ncell", type=int, default=4, help="diamond cells per edge (4 = Si512)")
     p.add_argument("--no-cpu-baseline", action="store_true")
-    p.add_argument("--cpu-local-bands", type=int, default=256,
-                   help="bands of the CPU baseline's local + kinetic term (>= nst: all)")
+    p.add_argument("--cpu-local-bands", type=int, default=1024,
+                   help="bands of the CPU baseline's local + kinetic term (>= nst: all; "
+                        "fewer are timed and scaled to nst)")
     p.add_argument("--no-fp32", action="store_true")
     p.add_argument("--no-batch", action="store_true", help="skip the 12-cell batch section")
     p.add_argument("--batch-reps", type=int, default=10)
